@@ -1,0 +1,174 @@
+/*
+ * lbmpsm — C ABI of the MI355X (gfx950) LBM-PSM fluid library behind
+ * `fix lbm-psm` and `fix lbm-psm-bc`.
+ *
+ * One context holds one lattice resident in device memory (populations,
+ * per-node particle slots, macroscopic fields) and one HIP stream.  All
+ * arguments are plain pointers and sizes; host arrays belong to the caller
+ * and are only read or written during the call.  Every entry point returns
+ * LBMPSM_OK (0) or a negative error code and never throws or exits;
+ * lbmpsm_last_error() gives the message.  One context per host thread;
+ * work is ordered on the context's stream and the lbmpsm_get_* calls
+ * synchronise it.
+ *
+ * Coordinates, velocities, angular velocities, radii, forces, torques and
+ * stresslets cross the boundary in the fix's physical units, exactly as
+ * LAMMPS holds them (contiguous [n][3] arrays such as atom->x[0]).  The unit
+ * conversion factors are those of the reference UnitConversion class
+ * (src/LBM_PSM_unitConversion.cpp:15-38); lbmpsm_config_from_fix() fills
+ * them from the fix arguments with the same formulas.
+ */
+#ifndef LBMPSM_H
+#define LBMPSM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBMPSM_OK 0
+#define LBMPSM_ERR_ARG (-1)      /* invalid argument                       */
+#define LBMPSM_ERR_HIP (-2)      /* HIP runtime error                      */
+#define LBMPSM_ERR_STATE (-3)    /* call not valid in the context's state  */
+#define LBMPSM_ERR_NOMEM (-4)    /* device or host allocation failed       */
+#define LBMPSM_ERR_NODEVICE (-5) /* no usable HIP device                   */
+#define LBMPSM_ERR_LIMIT (-6)    /* a documented size limit was exceeded   */
+
+/* Population storage forms. */
+#define LBMPSM_STORE_F64 0 /* FP64 populations, FP64 arithmetic (reference form) */
+#define LBMPSM_STORE_F32 1 /* FP32 f - w*rho0, FP32 arithmetic                   */
+#define LBMPSM_STORE_F16 2 /* FP16 (f - w*rho0)*2^s, FP32 arithmetic             */
+
+/* `fix lbm-psm-bc` keywords, src/fix_LBM_PSM_BC.cpp:24-39. */
+#define LBMPSM_BC_NONE 0
+#define LBMPSM_BC_SHEAR 1
+#define LBMPSM_BC_XFLOW 2
+#define LBMPSM_BC_CHANNEL 3
+#define LBMPSM_BC_CHANNEL_VELIN_PRESSOUT 4
+#define LBMPSM_BC_CLOSEDBOX 5
+
+/* lbmpsm_step() flags. */
+#define LBMPSM_STEP_STORE_MACROS 1 /* keep this step's rho and u for lbmpsm_get_macros */
+
+typedef struct {
+  int dim;           /* 2 (D2Q9) or 3 (D3Q19)                                 */
+  int nx, ny, nz;    /* global node counts without halo; nz ignored for dim 2 */
+  int periodic[3];   /* LAMMPS periodicity per axis                           */
+  double tau;        /* BGK relaxation time, > 0.5                            */
+  double F_lbm[3];   /* body force in lattice units                           */
+  double origin[3];  /* domain->boxlo (origin[2] is 0 in 2D)                  */
+  double dx;         /* UnitConversion::get_dx                                */
+  double Uc, u_lb;   /* velocity conversion  v_lb = v / Uc * u_lb             */
+  double tc, dt_d;   /* frequency conversion w_lb = w * tc * dt_d             */
+  double force_factor, torque_factor; /* lattice -> physical F and T, S      */
+  int storage;       /* LBMPSM_STORE_*                                        */
+  int device;        /* HIP device ordinal                                    */
+} lbmpsm_config;
+
+typedef struct lbmpsm_ctx lbmpsm_ctx;
+
+/* Lattice geometry as the context stores it. */
+typedef struct {
+  int dim, q;
+  int nx, ny, nz;          /* global interior node counts                   */
+  int sx, sy, sz;          /* stored extents (one halo layer kept on each
+                              non-periodic axis)                            */
+  int ref_offset[3];       /* reference halo'd index = stored index + this  */
+  int64_t stored_nodes;
+  int storage;
+  int64_t device_bytes;    /* device memory held by the context             */
+} lbmpsm_info;
+
+/* Fill `cfg` from the `fix lbm-psm` arguments (every Nlc lc rho nu Re tau
+ * Fext) and the LAMMPS box, with the reference's formulas:
+ * UnitConversion (src/LBM_PSM_unitConversion.cpp:15-38), node counts
+ * (src/fix_LBM_PSM.cpp:142-154) and the lattice body force
+ * (src/fix_LBM_PSM.cpp:161-162).  Also checks Ma <= 0.3 and that the box is
+ * an integer number of cells (src/fix_LBM_PSM.cpp:122-140).
+ * Replaces fix_LBM_PSM::init's set-up half, src/fix_LBM_PSM.cpp:104-170. */
+int lbmpsm_config_from_fix(lbmpsm_config* cfg, int dim, int Nlc, double lc, double rho,
+                           double nu, double Re, double tau, const double Fext[3],
+                           const double boxlo[3], const double prd[3],
+                           const int periodic[3]);
+
+/* Allocate the lattice on cfg->device.
+ * Replaces `new LBMPSMBGKDynamics(...)` + `initialise_domain`,
+ * src/fix_LBM_PSM.cpp:164-166. */
+int lbmpsm_create(const lbmpsm_config* cfg, lbmpsm_ctx** out);
+int lbmpsm_destroy(lbmpsm_ctx* ctx);
+
+/* Message of the last failed call on ctx (or of lbmpsm_create when ctx is
+ * NULL).  The pointer stays valid until the next call on that context. */
+const char* lbmpsm_last_error(const lbmpsm_ctx* ctx);
+
+int lbmpsm_get_info(const lbmpsm_ctx* ctx, lbmpsm_info* info);
+
+/* Equilibrium start and empty particle slots.
+ * Replaces LBMPSMBGKDynamics::initialise_dynamics,
+ * src/LBM_PSM_BGK_dynamics.cpp:22-44, with the slot reset of
+ * src/LBM_PSM_lattice.cpp:142-186. */
+int lbmpsm_init_equilibrium(lbmpsm_ctx* ctx, double rho, double ux, double uy, double uz);
+
+/* Select the `fix lbm-psm-bc` face set applied after every streaming step.
+ * u_infty is UnitConversion::get_u_lb(), rho_outlet 1.0 in the reference.
+ * Replaces fix_LBM_PSM_BC::post_force, src/fix_LBM_PSM_BC.cpp:151-402. */
+int lbmpsm_set_bc(lbmpsm_ctx* ctx, int keyword, double u_infty, double rho_outlet);
+
+/* Map n spheres (LAMMPS order, locals then ghosts) onto the lattice: solid
+ * fraction, rigid-body node velocity and the two-slot assignment.
+ * Replaces ExchangeParticleData::setParticlesOnLattice,
+ * src/LBM_PSM_exchangeParticleData.cpp:24-112. */
+int lbmpsm_set_particles(lbmpsm_ctx* ctx, int n, const int* tag, const double* x,
+                         const double* v, const double* omega, const double* radius);
+
+/* One LBM step: collide-stream of every node with PSM weighting and Guo
+ * forcing, the selected Zou-He faces, and the per-particle force, torque and
+ * stresslet reduction for the list last given to lbmpsm_set_particles.
+ * Replaces the LBM branch of fix_LBM_PSM::post_force,
+ * src/fix_LBM_PSM.cpp:206-229 (halo refresh, macroCollideStream and the
+ * calculateHydrodynamicInteractions loop), and fix_LBM_PSM_BC::post_force. */
+int lbmpsm_step(lbmpsm_ctx* ctx, int flags);
+
+/* Per-list-entry hydrodynamic force [n][3], torque [n][3] and stresslet
+ * [n][6] (xx yy zz xy xz yz) of the last step, physical units; any pointer
+ * may be NULL.  Replaces the fh/th/stresslet outputs of
+ * ExchangeParticleData::calculateHydrodynamicInteractions,
+ * src/LBM_PSM_exchangeParticleData.cpp:172-244. */
+int lbmpsm_get_hydro(lbmpsm_ctx* ctx, int n, double* F, double* T, double* S);
+
+/* Populations in the reference's post-stream, post-BC state and layout
+ * (src/LBM_PSM_lattice.h:90): interior nodes only, [nx][ny][nz][q] with
+ * the velocity index fastest (nz = 1 in 2D).
+ * Replaces LBMPSMLattice::getVector_f_curr / setVector_f_curr,
+ * src/LBM_PSM_lattice.cpp:250-252. */
+int lbmpsm_get_populations(lbmpsm_ctx* ctx, double* f);
+int lbmpsm_set_populations(lbmpsm_ctx* ctx, const double* f);
+
+/* rho [nx][ny][nz] and u [nx][ny][nz][3] of the last step run with
+ * LBMPSM_STEP_STORE_MACROS (reference get_rho_reference/get_u_reference,
+ * src/LBM_PSM_lattice.cpp:214-222, interior nodes). */
+int lbmpsm_get_macros(lbmpsm_ctx* ctx, double* rho, double* u);
+
+/* Per-node particle slots, interior nodes in reference order:
+ * tag [node][2], sf [node][2], u_s [node][6], F [node][6] (lattice units)
+ * (ParticleDataOnLattice, src/LBM_PSM_particleDataOnLattice.h:22-32);
+ * any pointer may be NULL. */
+int lbmpsm_get_slots(lbmpsm_ctx* ctx, int* tag, double* sf, double* us, double* F);
+
+/* Sum of all populations over interior nodes, accumulated in FP64. */
+int lbmpsm_mass(lbmpsm_ctx* ctx, double* mass);
+
+int lbmpsm_sync(lbmpsm_ctx* ctx);
+
+/* Per-kernel timing with HIP events on the context's stream. */
+int lbmpsm_profile_enable(lbmpsm_ctx* ctx, int enable);
+/* Total milliseconds and launch count recorded for a kernel name
+ * ("collide_stream", "map_particles", "hydro_reduce", ...). */
+int lbmpsm_profile_read(lbmpsm_ctx* ctx, const char* kernel, double* total_ms, int64_t* count);
+int lbmpsm_profile_reset(lbmpsm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,923 @@
+// C ABI of the lbmpsm library: context management, host-side set-up that
+// mirrors `fix lbm-psm` / `fix lbm-psm-bc`, and the launch sequence of one
+// LBM step.  See include/lbmpsm.h for the contract of each entry point.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "lbmpsm.h"
+#include "lbmpsm_internal.h"
+
+using namespace lbmpsm;
+
+namespace {
+
+std::string g_create_error;
+
+struct ProfEntry {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct lbmpsm_ctx {
+  lbmpsm_config cfg{};
+  Geom g{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // populations: two buffers, cur holds the current state
+  void* f[2] = {nullptr, nullptr};
+  size_t elem = 8;
+  int cur = 0;
+  int poststream = 1;
+  double cb_raw[19] = {0};     // initial equilibrium (never-written halo values)
+  double cb_store[19] = {0};   // same in storage form
+  double rho0 = 1.0;
+  float scale = 1.0f, inv_scale = 1.0f;
+
+  SlotsDev slots;
+  double* rho = nullptr;  // macros [plane], [3][plane]
+  double* u = nullptr;
+  int have_macros = 0;
+
+  ParticleDev part;
+  BrickDev br;
+  // pinned host staging for one particle upload and the hydro download
+  void* h_stage = nullptr;
+  size_t h_stage_bytes = 0;
+  hipEvent_t stage_done = nullptr;
+  double* h_out = nullptr;
+  size_t h_out_bytes = 0;
+  int hydro_ready = 0;
+  int hydro_n = 0;
+
+  // boundary faces
+  int bc_keyword = 0;
+  int nfaces = 0;
+  Face faces[kMaxFaces];
+
+  double* d_cb = nullptr;       // [19] cb_store on the device
+  Face* d_faces = nullptr;      // [kMaxFaces] on the device
+  double* d_scratch = nullptr;  // finish / mass scratch
+  size_t d_scratch_bytes = 0;
+
+  int64_t device_bytes = 0;
+
+  // profiling
+  int prof = 0;
+  std::vector<ProfEntry> pending;
+  std::vector<hipEvent_t> free_events;
+  std::map<std::string, std::pair<double, int64_t>> totals;
+};
+
+namespace {
+
+int fail(lbmpsm_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                              \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      return fail((ctx), LBMPSM_ERR_HIP,                                               \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));                  \
+  } while (0)
+
+hipEvent_t take_event(lbmpsm_ctx* c) {
+  if (!c->free_events.empty()) {
+    hipEvent_t e = c->free_events.back();
+    c->free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets a launch with events on the context stream when profiling.
+struct ProfScope {
+  lbmpsm_ctx* c;
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(lbmpsm_ctx* c_, const char* n) : c(c_), name(n) {
+    if (c->prof) {
+      a = take_event(c);
+      b = take_event(c);
+      if (a && b) (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~ProfScope() {
+    if (c->prof && a && b) {
+      (void)hipEventRecord(b, c->stream);
+      c->pending.push_back({name, a, b});
+    }
+  }
+};
+
+int prof_collect(lbmpsm_ctx* c) {
+  if (c->pending.empty()) return LBMPSM_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto& p : c->pending) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, p.a, p.b));
+    auto& t = c->totals[p.name];
+    t.first += ms;
+    t.second += 1;
+    c->free_events.push_back(p.a);
+    c->free_events.push_back(p.b);
+  }
+  c->pending.clear();
+  return LBMPSM_OK;
+}
+
+template <typename T>
+int dev_alloc(lbmpsm_ctx* c, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, count * sizeof(T));
+  if (e != hipSuccess)
+    return fail(c, LBMPSM_ERR_NOMEM,
+                "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed: " +
+                    hipGetErrorString(e));
+  *p = (T*)q;
+  c->device_bytes += (int64_t)(count * sizeof(T));
+  return LBMPSM_OK;
+}
+
+void dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+// Reference feq with a node's rho and u, src/LBM_PSM_dynamics.h:43-59.
+double feq_ref(int q, int iq, double rho, double ux, double uy, double uz) {
+  static const int ex19[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0};
+  static const int ey19[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1};
+  static const int ez19[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1};
+  static const int ex9[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+  static const int ey9[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+  const double csPow2 = 1.0 / 3.0;
+  const double csPow4 = csPow2 * csPow2;
+  const double invCsPow2 = 1.0 / csPow2;
+  const double invCsPow4 = 1.0 / csPow4;
+  double ex_i, ey_i, ez_i, w_i;
+  if (q == 19) {
+    ex_i = ex19[iq];
+    ey_i = ey19[iq];
+    ez_i = ez19[iq];
+    w_i = iq == 0 ? 1.0 / 3.0 : (iq < 7 ? 1.0 / 18.0 : 1.0 / 36.0);
+  } else {
+    ex_i = ex9[iq];
+    ey_i = ey9[iq];
+    ez_i = 0;
+    w_i = iq == 0 ? 4.0 / 9.0 : (iq < 5 ? 1.0 / 9.0 : 1.0 / 36.0);
+  }
+  double eiq_dot_u = ex_i * ux + ey_i * uy + ez_i * uz;
+  return rho * w_i *
+         (1.0 + eiq_dot_u * invCsPow2 + eiq_dot_u * eiq_dot_u * 0.5 * invCsPow4 -
+          0.5 * (ux * ux + uy * uy + uz * uz) * invCsPow2);
+}
+
+double wq(int q, int iq) {
+  if (q == 19) return iq == 0 ? 1.0 / 3.0 : (iq < 7 ? 1.0 / 18.0 : 1.0 / 36.0);
+  return iq == 0 ? 4.0 / 9.0 : (iq < 5 ? 1.0 / 9.0 : 1.0 / 36.0);
+}
+
+CollideArgs make_args(lbmpsm_ctx* c, int store_macros) {
+  CollideArgs a{};
+  a.g = c->g;
+  a.fin = c->f[c->cur];
+  a.fout = c->f[1 - c->cur];
+  a.mask = c->slots.mask;
+  a.sf = c->slots.sf;
+  a.us = c->slots.us;
+  a.Fh = c->slots.Fh;
+  a.rho_out = store_macros ? c->rho : nullptr;
+  a.u_out = store_macros ? c->u : nullptr;
+  a.poststream = c->poststream;
+  const double* F = c->cfg.F_lbm;
+  a.guo = (F[0] * F[0] + F[1] * F[1] + F[2] * F[2]) > 0.0 ? 1 : 0;
+  a.omega = 1.0 / c->cfg.tau;
+  a.guo_pref = 1.0 - 0.5 * a.omega;
+  a.F[0] = F[0];
+  a.F[1] = F[1];
+  a.F[2] = F[2];
+  const double csPow2 = 1.0 / 3.0;
+  const double csPow4 = csPow2 * csPow2;
+  a.invCs2 = 1.0 / csPow2;
+  a.invCs4 = 1.0 / csPow4;
+  a.rho0 = c->rho0;
+  a.scale = c->scale;
+  a.inv_scale = c->inv_scale;
+  a.cb = c->d_cb;
+  a.faces = c->d_faces;
+  a.nfaces = c->nfaces;
+  a.host_faces = c->faces;
+  return a;
+}
+
+int ensure_scratch(lbmpsm_ctx* c, size_t bytes) {
+  if (c->d_scratch_bytes >= bytes) return LBMPSM_OK;
+  if (c->d_scratch) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dev_free(c->d_scratch);
+    c->device_bytes -= (int64_t)c->d_scratch_bytes;
+    c->d_scratch = nullptr;
+    c->d_scratch_bytes = 0;
+  }
+  int rc = dev_alloc(c, (char**)&c->d_scratch, bytes);
+  if (rc) return rc;
+  c->d_scratch_bytes = bytes;
+  return LBMPSM_OK;
+}
+
+int ensure_host_out(lbmpsm_ctx* c, size_t bytes) {
+  if (c->h_out_bytes >= bytes) return LBMPSM_OK;
+  if (c->h_out) (void)hipHostFree(c->h_out);
+  c->h_out = nullptr;
+  c->h_out_bytes = 0;
+  HIPCHK(c, hipHostMalloc((void**)&c->h_out, bytes));
+  c->h_out_bytes = bytes;
+  return LBMPSM_OK;
+}
+
+int ensure_particles(lbmpsm_ctx* c, int n) {
+  ParticleDev& P = c->part;
+  if (n <= P.cap) return LBMPSM_OK;
+  // earlier steps may still read these buffers
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int cap = std::max(n, std::max(64, P.cap * 2));
+  void* old[] = {P.tag, P.x, P.v, P.w, P.r, P.xl, P.kin, P.zone, P.rzone, P.out};
+  for (void* p : old) dev_free(p);
+  int rc = 0;
+  rc |= dev_alloc(c, &P.tag, cap);
+  rc |= dev_alloc(c, &P.x, 3 * (size_t)cap);
+  rc |= dev_alloc(c, &P.v, 3 * (size_t)cap);
+  rc |= dev_alloc(c, &P.w, 3 * (size_t)cap);
+  rc |= dev_alloc(c, &P.r, cap);
+  rc |= dev_alloc(c, &P.xl, 4 * (size_t)cap);
+  rc |= dev_alloc(c, &P.kin, 6 * (size_t)cap);
+  rc |= dev_alloc(c, &P.zone, 6 * (size_t)cap);
+  rc |= dev_alloc(c, &P.rzone, 6 * (size_t)cap);
+  rc |= dev_alloc(c, &P.out, 12 * (size_t)cap);
+  if (rc) return LBMPSM_ERR_NOMEM;
+  P.cap = cap;
+  // pinned staging: tag + 10 doubles per entry
+  const size_t bytes = (size_t)cap * (sizeof(int) + 10 * sizeof(double));
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
+  c->h_stage = nullptr;
+  HIPCHK(c, hipHostMalloc(&c->h_stage, bytes));
+  c->h_stage_bytes = bytes;
+  return LBMPSM_OK;
+}
+
+// Face table of `fix lbm-psm-bc`, src/fix_LBM_PSM_BC.cpp:223-401, in the
+// reference's halo'd coordinates, converted to stored coordinates.
+void add_face(lbmpsm_ctx* c, int kind, int axis, int plane_ref, int a_lo, int a_hi, int b_lo,
+              int b_hi, double ux, double uy, double uz, double rho) {
+  const Geom& g = c->g;
+  const int ta = axis == 0 ? 1 : 0;         // first tangential axis
+  const int tb = axis == 2 ? 1 : 2;         // second tangential axis
+  Face F{};
+  F.kind = kind;
+  F.axis = axis;
+  F.plane = plane_ref - g.off[axis];
+  F.a_lo = a_lo - g.off[ta];
+  F.a_hi = a_hi - g.off[ta];
+  F.b_lo = b_lo - g.off[tb];
+  F.b_hi = b_hi - g.off[tb];
+  F.u[0] = ux;
+  F.u[1] = uy;
+  F.u[2] = uz;
+  F.rho = rho;
+  c->faces[c->nfaces++] = F;
+}
+
+void build_faces(lbmpsm_ctx* c, int typeBC, double U, double rho_outlet) {
+  c->nfaces = 0;
+  const int e = 1;
+  const int nx = c->g.nx + 2, ny = c->g.ny + 2;  // reference halo'd sizes
+  const int nz = c->g.dim == 3 ? c->g.nz + 2 : 1;
+  if (c->g.dim == 2) {
+    // 2D faces have no z range: node k = 0 only.
+    auto xn = [&](double u) { add_face(c, FK_2D_VEL_XN, 0, e, e, ny - 1 - e, 0, 0, u, 0, 0, 0); };
+    auto xp = [&](double u) { add_face(c, FK_2D_VEL_XP, 0, nx - 1 - e, e, ny - 1 - e, 0, 0, u, 0, 0, 0); };
+    auto dxp = [&](double r) { add_face(c, FK_2D_DENS_XP, 0, nx - 1 - e, e, ny - 1 - e, 0, 0, 0, 0, 0, r); };
+    auto yn = [&](int i0, int i1, double u) { add_face(c, FK_2D_VEL_YN, 1, 0 + e, i0, i1, 0, 0, u, 0, 0, 0); };
+    auto yp = [&](int i0, int i1, double u) { add_face(c, FK_2D_VEL_YP, 1, ny - 1 - e, i0, i1, 0, 0, u, 0, 0, 0); };
+    switch (typeBC) {
+      case 1: yn(e, nx - 1 - e, -U); yp(e, nx - 1 - e, U); break;
+      case 2: xn(U); dxp(rho_outlet); yn(1 + e, nx - 2 - e, U); yp(1 + e, nx - 2 - e, U); break;
+      case 3: yn(e, nx - 1 - e, 0.0); yp(e, nx - 1 - e, 0.0); break;
+      case 4: xn(U); dxp(rho_outlet); yn(e, nx - 1 - e, 0.0); yp(e, nx - 1 - e, 0.0); break;
+      case 5: xn(0.0); xp(0.0); yn(e, nx - 1 - e, 0.0); yp(e, nx - 1 - e, 0.0); break;
+      default: break;
+    }
+    return;
+  }
+  auto xn = [&](double ux) { add_face(c, FK_3D_VEL_XN, 0, e, e, ny - 1 - e, e, nz - 1 - e, ux, 0, 0, 0); };
+  auto xp = [&](double ux) { add_face(c, FK_3D_VEL_XP, 0, nx - 1 - e, e, ny - 1 - e, e, nz - 1 - e, ux, 0, 0, 0); };
+  auto dxp = [&](double r) { add_face(c, FK_3D_DENS_XP, 0, nx - 1 - e, e, ny - 1 - e, e, nz - 1 - e, 0, 0, 0, r); };
+  auto yn = [&](double ux) { add_face(c, FK_3D_VEL_YN, 1, e, e, nx - 1 - e, e, nz - 1 - e, ux, 0, 0, 0); };
+  auto yp = [&](double ux) { add_face(c, FK_3D_VEL_YP, 1, ny - 1 - e, e, nx - 1 - e, e, nz - 1 - e, ux, 0, 0, 0); };
+  auto zn = [&](double ux) { add_face(c, FK_3D_VEL_ZN, 2, e, e, nx - 1 - e, e, ny - 1 - e, ux, 0, 0, 0); };
+  auto zp = [&](double ux) { add_face(c, FK_3D_VEL_ZP, 2, nz - 1 - e, e, nx - 1 - e, e, ny - 1 - e, ux, 0, 0, 0); };
+  switch (typeBC) {
+    case 1: yn(-U); yp(U); break;
+    case 2: xn(U); dxp(rho_outlet); yn(U); yp(U); zn(U); zp(U); break;
+    case 3: yn(0.0); yp(0.0); break;
+    case 4: xn(U); dxp(rho_outlet); yn(0.0); yp(0.0); zn(0.0); zp(0.0); break;
+    case 5: xn(0.0); xp(0.0); yn(0.0); yp(0.0); zn(0.0); zp(0.0); break;
+    default: break;
+  }
+}
+
+int run_map(lbmpsm_ctx* c) {
+  ParticleDev& P = c->part;
+  BrickDev& B = c->br;
+  PrepArgs pa{};
+  pa.g = c->g;
+  pa.n = P.n;
+  pa.x = P.x;
+  pa.v = P.v;
+  pa.w = P.w;
+  pa.r = P.r;
+  pa.xl = P.xl;
+  pa.kin = P.kin;
+  pa.zone = P.zone;
+  pa.rzone = P.rzone;
+  for (int d = 0; d < 3; ++d) pa.origin[d] = c->cfg.origin[d];
+  pa.dx = c->cfg.dx;
+  pa.Uc = c->cfg.Uc;
+  pa.u_lb = c->cfg.u_lb;
+  pa.tc = c->cfg.tc;
+  pa.dt_d = c->cfg.dt_d;
+  pa.count = B.count;
+  pa.bi = B.bi;
+  pa.bj = B.bj;
+  pa.bk = B.bk;
+  pa.nbi = B.nbi;
+  pa.nbj = B.nbj;
+  pa.nbk = B.nbk;
+  {
+    ProfScope ps(c, "map_bin");
+    HIPCHK(c, launch_fill_int(B.count, B.nbricks, 0, c->stream));
+    HIPCHK(c, launch_prep_count(pa, c->stream));
+    HIPCHK(c, launch_scan_bricks(B, c->stream));
+    HIPCHK(c, launch_fill_lists(pa, B, c->stream));
+  }
+  MapArgs ma{};
+  ma.g = c->g;
+  ma.s = c->slots;
+  ma.tag = P.tag;
+  ma.xl = P.xl;
+  ma.kin = P.kin;
+  ma.zone = P.zone;
+  ma.list = B.list;
+  ma.offset = B.offset;
+  ma.count = B.count;
+  ma.nonempty = B.nonempty;
+  ma.nonempty_n = B.nonempty_n;
+  ma.bi = B.bi;
+  ma.bj = B.bj;
+  ma.bk = B.bk;
+  ma.nbi = B.nbi;
+  ma.nbj = B.nbj;
+  ma.nbk = B.nbk;
+  ma.hsq_half = c->g.dim == 3 ? std::sqrt(3.1) / 2.0 : std::sqrt(2.1) / 2.0;
+  const int grid = (int)std::min<int64_t>(B.nbricks, 4096);
+  {
+    ProfScope ps(c, "map_particles");
+    HIPCHK(c, launch_map(ma, B.overflow, c->stream, grid));
+  }
+  return LBMPSM_OK;
+}
+
+int check_overflow(lbmpsm_ctx* c) {
+  int ov = 0;
+  HIPCHK(c, hipMemcpyAsync(&ov, c->br.overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (ov)
+    return fail(c, LBMPSM_ERR_LIMIT,
+                "more than 2048 particle zones overlap one 256-node brick; the map was not applied");
+  return LBMPSM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lbmpsm_last_error(const lbmpsm_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int lbmpsm_config_from_fix(lbmpsm_config* cfg, int dim, int Nlc, double lc, double rho,
+                           double nu, double Re, double tau, const double Fext[3],
+                           const double boxlo[3], const double prd[3], const int periodic[3]) {
+  if (!cfg || !Fext || !boxlo || !prd || !periodic) return fail(nullptr, LBMPSM_ERR_ARG, "null argument");
+  if (dim != 2 && dim != 3) return fail(nullptr, LBMPSM_ERR_ARG, "dimension must be 2 or 3");
+  if (Nlc <= 1 || lc <= 0.0 || rho <= 0.0 || nu <= 0.0 || Re <= 0.0)
+    return fail(nullptr, LBMPSM_ERR_ARG, "Illegal fix lbm-psm command");
+  if (tau <= 0.5) return fail(nullptr, LBMPSM_ERR_ARG, "Illegal fix lbm-psm command: tau <= 0.5");
+  std::memset(cfg, 0, sizeof(*cfg));
+  // UnitConversion, src/LBM_PSM_unitConversion.cpp:15-38
+  const double Uc_d = 1.0, Lc_d = 1.0;
+  const double csPow2 = 1.0 / 3.0;
+  const double nu_lb = (tau - 0.5) * csPow2;
+  const double dx = lc / (double)(Nlc - 1);
+  const double dx_d = Lc_d / (double)(Nlc - 1);
+  const double Uc = Re * nu / lc;
+  const double u_lb = Re * nu_lb / (double)(Nlc - 1);
+  const double dt_d = dx_d * u_lb / Uc_d;
+  const double tc = lc / Uc;
+  const double forceFactor = rho * std::pow(lc, 4) / std::pow(tc, 2) * std::pow(dx_d, 4) / std::pow(dt_d, 2);
+  const double torqueFactor = rho * std::pow(lc, 5) / std::pow(tc, 2) * std::pow(dx_d, 5) / std::pow(dt_d, 2);
+  const double volumeForceFactor = rho * lc / std::pow(tc, 2) * dx_d / std::pow(dt_d, 2);
+  // src/fix_LBM_PSM.cpp:122-124
+  const double Ma = u_lb / (1.0 / std::sqrt(3.0));
+  if (Ma > 0.3)
+    return fail(nullptr, LBMPSM_ERR_ARG,
+                "Mach number is Ma > 0.3. Aborting simulation. Choose parameters which give Ma < 0.3 for stability reasons.");
+  // src/fix_LBM_PSM.cpp:135-140
+  const double SMALL = 1e-15;
+  for (int d = 0; d < dim; ++d) {
+    double r = prd[d] / dx;
+    if (std::pow(((double)((int)(r + 0.5)) - r), 2.0) > SMALL)
+      return fail(nullptr, LBMPSM_ERR_ARG,
+                  "Illegal cell width. Division of domain length and cell width has to give an integer.");
+  }
+  // src/fix_LBM_PSM.cpp:142-154
+  int n[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) {
+    n[d] = (int)(prd[d] / dx + 1.5);
+    if (periodic[d]) n[d] -= 1;
+  }
+  cfg->dim = dim;
+  cfg->nx = n[0];
+  cfg->ny = n[1];
+  cfg->nz = dim == 3 ? n[2] : 1;
+  for (int d = 0; d < 3; ++d) cfg->periodic[d] = periodic[d] ? 1 : 0;
+  if (dim == 2) cfg->periodic[2] = 1;
+  cfg->tau = tau;
+  // get_volume_force_lb, src/LBM_PSM_unitConversion.cpp:81-88
+  for (int d = 0; d < 3; ++d) cfg->F_lbm[d] = Fext[d] / volumeForceFactor;
+  cfg->origin[0] = boxlo[0];
+  cfg->origin[1] = boxlo[1];
+  cfg->origin[2] = dim == 2 ? 0.0 : boxlo[2];
+  cfg->dx = dx;
+  cfg->Uc = Uc;
+  cfg->u_lb = u_lb;
+  cfg->tc = tc;
+  cfg->dt_d = dt_d;
+  cfg->force_factor = forceFactor;
+  cfg->torque_factor = torqueFactor;
+  cfg->storage = LBMPSM_STORE_F64;
+  cfg->device = 0;
+  return LBMPSM_OK;
+}
+
+int lbmpsm_create(const lbmpsm_config* cfg, lbmpsm_ctx** out) {
+  if (!cfg || !out) return fail(nullptr, LBMPSM_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (cfg->dim != 2 && cfg->dim != 3) return fail(nullptr, LBMPSM_ERR_ARG, "dim must be 2 or 3");
+  if (cfg->nx < 2 || cfg->ny < 2 || (cfg->dim == 3 && cfg->nz < 2))
+    return fail(nullptr, LBMPSM_ERR_ARG, "need at least 2 nodes per axis");
+  if (!(cfg->tau > 0.5)) return fail(nullptr, LBMPSM_ERR_ARG, "tau must be > 0.5");
+  if (cfg->storage < LBMPSM_STORE_F64 || cfg->storage > LBMPSM_STORE_F16)
+    return fail(nullptr, LBMPSM_ERR_ARG, "unknown storage form");
+  if (!(cfg->dx > 0.0) || !(cfg->Uc != 0.0))
+    return fail(nullptr, LBMPSM_ERR_ARG, "unit conversion factors not set");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return fail(nullptr, LBMPSM_ERR_NODEVICE, "no HIP device available");
+  if (cfg->device < 0 || cfg->device >= ndev)
+    return fail(nullptr, LBMPSM_ERR_ARG, "device ordinal out of range");
+
+  lbmpsm_ctx* c = new (std::nothrow) lbmpsm_ctx();
+  if (!c) return fail(nullptr, LBMPSM_ERR_NOMEM, "host allocation failed");
+  c->cfg = *cfg;
+  c->device = cfg->device;
+  if (hipSetDevice(c->device) != hipSuccess) {
+    delete c;
+    return fail(nullptr, LBMPSM_ERR_HIP, "hipSetDevice failed");
+  }
+  Geom& g = c->g;
+  g.dim = cfg->dim;
+  g.q = cfg->dim == 3 ? 19 : 9;
+  g.nx = cfg->nx;
+  g.ny = cfg->ny;
+  g.nz = cfg->dim == 3 ? cfg->nz : 1;
+  const int glob[3] = {g.nx, g.ny, g.nz};
+  int s[3];
+  for (int d = 0; d < 3; ++d) {
+    const bool active = d < cfg->dim;
+    const bool per = cfg->periodic[d] != 0;
+    g.per[d] = (!active || per) ? 1 : 0;
+    if (!active) {
+      s[d] = 1;
+      g.off[d] = 0;
+    } else if (per) {
+      s[d] = glob[d];
+      g.off[d] = 1;
+    } else {
+      s[d] = glob[d] + 2;
+      g.off[d] = 0;
+    }
+  }
+  g.sx = s[0];
+  g.sy = s[1];
+  g.sz = s[2];
+  g.nodes = (int64_t)s[0] * s[1] * s[2];
+  g.plane = (g.nodes + 63) / 64 * 64;
+  if ((int64_t)g.plane >= (int64_t)1 << 31) {
+    delete c;
+    return fail(nullptr, LBMPSM_ERR_LIMIT, "lattice exceeds 2^31 stored nodes");
+  }
+  c->elem = cfg->storage == LBMPSM_STORE_F64 ? 8 : (cfg->storage == LBMPSM_STORE_F32 ? 4 : 2);
+  if (cfg->storage == LBMPSM_STORE_F16) {
+    c->scale = 4096.0f;
+    c->inv_scale = 1.0f / 4096.0f;
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(nullptr, LBMPSM_ERR_HIP, "hipStreamCreate failed");
+  }
+  int rc = 0;
+  const size_t P = (size_t)g.plane;
+  rc |= dev_alloc(c, (char**)&c->f[0], P * g.q * c->elem);
+  rc |= dev_alloc(c, (char**)&c->f[1], P * g.q * c->elem);
+  rc |= dev_alloc(c, &c->slots.tag, 2 * P);
+  rc |= dev_alloc(c, &c->slots.sf, 2 * P);
+  rc |= dev_alloc(c, &c->slots.us, 6 * P);
+  rc |= dev_alloc(c, &c->slots.Fh, 6 * P);
+  rc |= dev_alloc(c, &c->slots.mask, P);
+  rc |= dev_alloc(c, &c->d_cb, 19);
+  rc |= dev_alloc(c, &c->d_faces, kMaxFaces);
+  // node bricks for the particle map: 256 nodes each
+  BrickDev& B = c->br;
+  if (g.dim == 3) {
+    B.bi = 4; B.bj = 4; B.bk = 16;
+  } else {
+    B.bi = 16; B.bj = 16; B.bk = 1;
+  }
+  B.nbi = (g.sx + B.bi - 1) / B.bi;
+  B.nbj = (g.sy + B.bj - 1) / B.bj;
+  B.nbk = (g.sz + B.bk - 1) / B.bk;
+  B.nbricks = (int64_t)B.nbi * B.nbj * B.nbk;
+  rc |= dev_alloc(c, &B.count, B.nbricks + 1);
+  rc |= dev_alloc(c, &B.offset, B.nbricks + 1);
+  rc |= dev_alloc(c, &B.cursor, B.nbricks + 1);
+  rc |= dev_alloc(c, &B.nonempty, B.nbricks + 1);
+  rc |= dev_alloc(c, &B.nonempty_n, 1);
+  rc |= dev_alloc(c, &B.block_sums, (B.nbricks + 1023) / 1024 + 1);
+  rc |= dev_alloc(c, &B.overflow, 1);
+  if (rc) {
+    std::string m = c->err;
+    lbmpsm_destroy(c);
+    return fail(nullptr, LBMPSM_ERR_NOMEM, m);
+  }
+  if (hipEventCreateWithFlags(&c->stage_done, hipEventDisableTiming) != hipSuccess) {
+    lbmpsm_destroy(c);
+    return fail(nullptr, LBMPSM_ERR_HIP, "hipEventCreate failed");
+  }
+  if (launch_fill_int(B.overflow, 1, 0, c->stream) != hipSuccess) {
+    lbmpsm_destroy(c);
+    return fail(nullptr, LBMPSM_ERR_HIP, "initial fill failed");
+  }
+  *out = c;
+  // reference start: rho = 1, u = 0 (src/fix_LBM_PSM.cpp:168)
+  rc = lbmpsm_init_equilibrium(c, 1.0, 0.0, 0.0, 0.0);
+  if (rc) {
+    g_create_error = c->err;
+    lbmpsm_destroy(c);
+    *out = nullptr;
+    return rc;
+  }
+  return LBMPSM_OK;
+}
+
+int lbmpsm_destroy(lbmpsm_ctx* c) {
+  if (!c) return LBMPSM_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  dev_free(c->f[0]);
+  dev_free(c->f[1]);
+  dev_free(c->slots.tag);
+  dev_free(c->slots.sf);
+  dev_free(c->slots.us);
+  dev_free(c->slots.Fh);
+  dev_free(c->slots.mask);
+  dev_free(c->rho);
+  dev_free(c->u);
+  ParticleDev& P = c->part;
+  void* pp[] = {P.tag, P.x, P.v, P.w, P.r, P.xl, P.kin, P.zone, P.rzone, P.out};
+  for (void* p : pp) dev_free(p);
+  BrickDev& B = c->br;
+  void* bb[] = {B.count, B.offset, B.cursor, B.nonempty, B.nonempty_n, B.block_sums, B.list, B.overflow};
+  for (void* p : bb) dev_free(p);
+  dev_free(c->d_scratch);
+  dev_free(c->d_cb);
+  dev_free(c->d_faces);
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
+  if (c->h_out) (void)hipHostFree(c->h_out);
+  for (auto& p : c->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : c->free_events) (void)hipEventDestroy(e);
+  if (c->stage_done) (void)hipEventDestroy(c->stage_done);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return LBMPSM_OK;
+}
+
+int lbmpsm_get_info(const lbmpsm_ctx* c, lbmpsm_info* info) {
+  if (!c || !info) return LBMPSM_ERR_ARG;
+  info->dim = c->g.dim;
+  info->q = c->g.q;
+  info->nx = c->g.nx;
+  info->ny = c->g.ny;
+  info->nz = c->g.nz;
+  info->sx = c->g.sx;
+  info->sy = c->g.sy;
+  info->sz = c->g.sz;
+  for (int d = 0; d < 3; ++d) info->ref_offset[d] = c->g.off[d];
+  info->stored_nodes = c->g.nodes;
+  info->storage = c->cfg.storage;
+  info->device_bytes = c->device_bytes;
+  return LBMPSM_OK;
+}
+
+int lbmpsm_init_equilibrium(lbmpsm_ctx* c, double rho, double ux, double uy, double uz) {
+  if (!c) return LBMPSM_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int Q = c->g.q;
+  for (int q = 0; q < Q; ++q) {
+    c->cb_raw[q] = feq_ref(Q, q, rho, ux, uy, uz);
+    c->cb_store[q] = c->cfg.storage == LBMPSM_STORE_F64 ? c->cb_raw[q]
+                                                        : c->cb_raw[q] - wq(Q, q) * c->rho0;
+  }
+  c->cur = 0;
+  HIPCHK(c, hipMemcpyAsync(c->d_cb, c->cb_store, sizeof(c->cb_store), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, launch_fill_planes(c->cfg.storage, c->f[0], c->g, c->cb_store, c->rho0, c->scale, c->stream));
+  HIPCHK(c, launch_slots_reset(c->g, c->slots, c->stream));
+  c->poststream = 1;
+  c->hydro_ready = 0;
+  c->have_macros = 0;
+  return LBMPSM_OK;
+}
+
+int lbmpsm_set_bc(lbmpsm_ctx* c, int keyword, double u_infty, double rho_outlet) {
+  if (!c) return LBMPSM_ERR_ARG;
+  if (keyword < LBMPSM_BC_NONE || keyword > LBMPSM_BC_CLOSEDBOX)
+    return fail(c, LBMPSM_ERR_ARG, "Illegal fix lbm-psm-bc command");
+  HIPCHK(c, hipSetDevice(c->device));
+  c->bc_keyword = keyword;
+  build_faces(c, keyword, u_infty, rho_outlet);
+  if (c->nfaces)
+    HIPCHK(c, hipMemcpyAsync(c->d_faces, c->faces, c->nfaces * sizeof(Face), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LBMPSM_OK;
+}
+
+int lbmpsm_set_particles(lbmpsm_ctx* c, int n, const int* tag, const double* x, const double* v,
+                         const double* omega, const double* radius) {
+  if (!c) return LBMPSM_ERR_ARG;
+  if (n < 0) return fail(c, LBMPSM_ERR_ARG, "negative particle count");
+  if (n > 0 && (!tag || !x || !v || !omega || !radius)) return fail(c, LBMPSM_ERR_ARG, "null particle array");
+  HIPCHK(c, hipSetDevice(c->device));
+  c->hydro_ready = 0;
+  c->part.n = n;
+  if (n == 0) return LBMPSM_OK;
+  int rc = ensure_particles(c, n);
+  if (rc) return rc;
+  // bound on (entry, brick) pairs for the list buffer
+  BrickDev& B = c->br;
+  int64_t pairs = 0;
+  const int bmin = std::min(B.bi, std::min(B.bj, B.bk));
+  for (int p = 0; p < n; ++p) {
+    const double span = 2.0 * std::fabs(radius[p]) / c->cfg.dx + 8.0;
+    if (!(span < 1e9)) return fail(c, LBMPSM_ERR_ARG, "radius is not finite");
+    int64_t per[3];
+    const int be[3] = {B.bi, B.bj, B.bk};
+    const int nb[3] = {B.nbi, B.nbj, B.nbk};
+    for (int d = 0; d < 3; ++d) per[d] = std::min<int64_t>((int64_t)(span / be[d]) + 2, nb[d]);
+    pairs += per[0] * per[1] * per[2];
+    (void)bmin;
+  }
+  if (pairs > B.list_cap) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int64_t cap = std::max<int64_t>(pairs, 2 * B.list_cap);
+    dev_free(B.list);
+    c->device_bytes -= B.list_cap * (int64_t)sizeof(int);
+    B.list = nullptr;
+    B.list_cap = 0;
+    rc = dev_alloc(c, &B.list, (size_t)cap);
+    if (rc) return rc;
+    B.list_cap = cap;
+  }
+  // stage into pinned memory, then one async copy per array
+  HIPCHK(c, hipEventSynchronize(c->stage_done));
+  char* h = (char*)c->h_stage;
+  int* ht = (int*)h;
+  double* hd = (double*)(h + (size_t)c->part.cap * sizeof(int));
+  std::memcpy(ht, tag, n * sizeof(int));
+  std::memcpy(hd, x, 3 * n * sizeof(double));
+  std::memcpy(hd + 3 * (size_t)n, v, 3 * n * sizeof(double));
+  std::memcpy(hd + 6 * (size_t)n, omega, 3 * n * sizeof(double));
+  std::memcpy(hd + 9 * (size_t)n, radius, n * sizeof(double));
+  ParticleDev& P = c->part;
+  HIPCHK(c, hipMemcpyAsync(P.tag, ht, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(P.x, hd, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(P.v, hd + 3 * (size_t)n, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(P.w, hd + 6 * (size_t)n, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(P.r, hd + 9 * (size_t)n, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipEventRecord(c->stage_done, c->stream));
+  return run_map(c);
+}
+
+int lbmpsm_step(lbmpsm_ctx* c, int flags) {
+  if (!c) return LBMPSM_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int store = (flags & LBMPSM_STEP_STORE_MACROS) ? 1 : 0;
+  if (store && !c->rho) {
+    int rc = dev_alloc(c, &c->rho, (size_t)c->g.plane);
+    rc |= dev_alloc(c, &c->u, 3 * (size_t)c->g.plane);
+    if (rc) return LBMPSM_ERR_NOMEM;
+  }
+  CollideArgs a = make_args(c, store);
+  {
+    ProfScope ps(c, "collide_stream");
+    HIPCHK(c, launch_collide(c->cfg.storage, a, c->stream));
+  }
+  c->cur = 1 - c->cur;
+  c->poststream = 0;
+  if (store) c->have_macros = 1;
+  HydroArgs h{};
+  h.g = c->g;
+  h.n = c->part.n;
+  h.tag = c->part.tag;
+  h.xl = c->part.xl;
+  h.rzone = c->part.rzone;
+  h.slot_tag = c->slots.tag;
+  h.Fh = c->slots.Fh;
+  h.forceFactor = c->cfg.force_factor;
+  h.torqueFactor = c->cfg.torque_factor;
+  h.out = c->part.out;
+  if (h.n > 0) {
+    ProfScope ps(c, "hydro_reduce");
+    HIPCHK(c, launch_hydro(h, c->stream));
+  }
+  c->hydro_ready = 1;
+  c->hydro_n = c->part.n;
+  return LBMPSM_OK;
+}
+
+int lbmpsm_get_hydro(lbmpsm_ctx* c, int n, double* F, double* T, double* S) {
+  if (!c) return LBMPSM_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->hydro_ready) return fail(c, LBMPSM_ERR_STATE, "no step has run since the particles were set");
+  if (n != c->hydro_n) return fail(c, LBMPSM_ERR_ARG, "n differs from the particle count of the last step");
+  int rc = check_overflow(c);
+  if (rc) return rc;
+  if (n == 0) return LBMPSM_OK;
+  rc = ensure_host_out(c, (size_t)n * 12 * sizeof(double));
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->h_out, c->part.out, (size_t)n * 12 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int p = 0; p < n; ++p) {
+    const double* o = c->h_out + 12 * (size_t)p;
+    if (F) for (int d = 0; d < 3; ++d) F[3 * p + d] = o[d];
+    if (T) for (int d = 0; d < 3; ++d) T[3 * p + d] = o[3 + d];
+    if (S) for (int d = 0; d < 6; ++d) S[6 * p + d] = o[6 + d];
+  }
+  return LBMPSM_OK;
+}
+
+int lbmpsm_get_populations(lbmpsm_ctx* c, double* f) {
+  if (!c || !f) return LBMPSM_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t count = (size_t)c->g.nx * c->g.ny * c->g.nz * c->g.q;
+  int rc = ensure_scratch(c, count * sizeof(double));
+  if (rc) return rc;
+  CollideArgs a = make_args(c, 0);
+  HIPCHK(c, launch_finish(c->cfg.storage, a, c->d_scratch, c->stream));
+  HIPCHK(c, hipMemcpyAsync(f, c->d_scratch, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LBMPSM_OK;
+}
+
+int lbmpsm_set_populations(lbmpsm_ctx* c, const double* f) {
+  if (!c || !f) return LBMPSM_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t count = (size_t)c->g.nx * c->g.ny * c->g.nz * c->g.q;
+  int rc = ensure_scratch(c, count * sizeof(double));
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->d_scratch, f, count * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  // halo nodes of non-periodic axes restart from the initial equilibrium
+  HIPCHK(c, launch_fill_planes(c->cfg.storage, c->f[c->cur], c->g, c->cb_store, c->rho0, c->scale, c->stream));
+  HIPCHK(c, launch_set_populations(c->cfg.storage, c->f[c->cur], c->g, c->d_scratch, c->rho0, c->scale, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->poststream = 1;
+  return LBMPSM_OK;
+}
+
+int lbmpsm_get_macros(lbmpsm_ctx* c, double* rho, double* u) {
+  if (!c) return LBMPSM_ERR_ARG;
+  if (!c->have_macros) return fail(c, LBMPSM_ERR_STATE, "no step stored macroscopic fields");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t nn = (size_t)c->g.nx * c->g.ny * c->g.nz;
+  int rc = ensure_scratch(c, nn * 4 * sizeof(double));
+  if (rc) return rc;
+  double* dr = c->d_scratch;
+  double* du = c->d_scratch + nn;
+  HIPCHK(c, launch_macros_export(c->g, c->rho, c->u, dr, du, c->stream));
+  if (rho) HIPCHK(c, hipMemcpyAsync(rho, dr, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (u) HIPCHK(c, hipMemcpyAsync(u, du, 3 * nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LBMPSM_OK;
+}
+
+int lbmpsm_get_slots(lbmpsm_ctx* c, int* tag, double* sf, double* us, double* F) {
+  if (!c) return LBMPSM_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t nn = (size_t)c->g.nx * c->g.ny * c->g.nz;
+  // scratch: tag (2 ints per node, stored as doubles' space) + 14 doubles
+  int rc = ensure_scratch(c, nn * (2 * sizeof(int) + 14 * sizeof(double)) + 64);
+  if (rc) return rc;
+  double* dsf = c->d_scratch;
+  double* dus = dsf + 2 * nn;
+  double* dF = dus + 6 * nn;
+  int* dtag = (int*)(dF + 6 * nn);
+  HIPCHK(c, launch_slots_export(c->g, c->slots, dtag, dsf, dus, dF, c->stream));
+  if (tag) HIPCHK(c, hipMemcpyAsync(tag, dtag, 2 * nn * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (sf) HIPCHK(c, hipMemcpyAsync(sf, dsf, 2 * nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (us) HIPCHK(c, hipMemcpyAsync(us, dus, 6 * nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (F) HIPCHK(c, hipMemcpyAsync(F, dF, 6 * nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LBMPSM_OK;
+}
+
+int lbmpsm_mass(lbmpsm_ctx* c, double* mass) {
+  if (!c || !mass) return LBMPSM_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int nb = (int)std::min<int64_t>(1024, (c->g.nodes + 255) / 256);
+  int rc = ensure_scratch(c, (size_t)nb * sizeof(double));
+  if (rc) return rc;
+  CollideArgs a = make_args(c, 0);
+  HIPCHK(c, launch_mass(c->cfg.storage, a, c->d_scratch, nb, c->stream));
+  std::vector<double> part(nb);
+  HIPCHK(c, hipMemcpyAsync(part.data(), c->d_scratch, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double m = 0.0;
+  for (int i = 0; i < nb; ++i) m += part[i];
+  *mass = m;
+  return LBMPSM_OK;
+}
+
+int lbmpsm_sync(lbmpsm_ctx* c) {
+  if (!c) return LBMPSM_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LBMPSM_OK;
+}
+
+int lbmpsm_profile_enable(lbmpsm_ctx* c, int enable) {
+  if (!c) return LBMPSM_ERR_ARG;
+  c->prof = enable ? 1 : 0;
+  return LBMPSM_OK;
+}
+
+int lbmpsm_profile_read(lbmpsm_ctx* c, const char* kernel, double* total_ms, int64_t* count) {
+  if (!c || !kernel) return LBMPSM_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = prof_collect(c);
+  if (rc) return rc;
+  auto it = c->totals.find(kernel);
+  if (total_ms) *total_ms = it == c->totals.end() ? 0.0 : it->second.first;
+  if (count) *count = it == c->totals.end() ? 0 : it->second.second;
+  return LBMPSM_OK;
+}
+
+int lbmpsm_profile_reset(lbmpsm_ctx* c) {
+  if (!c) return LBMPSM_ERR_ARG;
+  int rc = prof_collect(c);
+  c->totals.clear();
+  return rc;
+}
+
+}  // extern "C"

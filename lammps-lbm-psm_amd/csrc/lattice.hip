@@ -1,0 +1,743 @@
+// Lattice kernels: fused pull-stream + Zou-He + BGK-PSM collide, and the
+// helpers that read the lattice back in the reference's form.
+//
+// State between steps is the post-collision population of every stored node
+// (structure of arrays, one plane per direction).  A step pulls
+// f_q(x) = S_q(x - e_q), applies the Zou-He faces of the previous step to
+// the pulled node, computes rho/u, collides and writes S'(x) to the other
+// buffer.  This is the reference's order (collide, push-stream, BC at the
+// destination; src/LBM_PSM_BGK_dynamics.cpp:81-109, src/fix_LBM_PSM_BC.cpp)
+// regrouped per node so that every population is read once and written once.
+//
+// Storage forms: F64 keeps raw populations and the reference's FP64
+// arithmetic (this TU is compiled with -ffp-contract=off).  F32 and F16 keep
+// g = f - w*rho0 (F16 scaled by a power of two) and compute in FP32 with
+// contraction enabled locally.
+#include <hip/hip_fp16.h>
+
+#include "lbmpsm_internal.h"
+
+namespace lbmpsm {
+
+enum { ST_F64 = 0, ST_F32 = 1, ST_F16 = 2 };
+
+template <int ST>
+struct StoreT;
+template <>
+struct StoreT<ST_F64> {
+  using T = double;
+  using C = double;
+};
+template <>
+struct StoreT<ST_F32> {
+  using T = float;
+  using C = float;
+};
+template <>
+struct StoreT<ST_F16> {
+  using T = __half;
+  using C = float;
+};
+
+template <int ST>
+__device__ __forceinline__ typename StoreT<ST>::C ld(const typename StoreT<ST>::T* p,
+                                                      float inv_scale) {
+  if constexpr (ST == ST_F16) {
+    return __half2float(*p) * inv_scale;
+  } else {
+    (void)inv_scale;
+    return *p;
+  }
+}
+
+template <int ST>
+__device__ __forceinline__ void st(typename StoreT<ST>::T* p, typename StoreT<ST>::C v,
+                                   float scale) {
+  if constexpr (ST == ST_F16) {
+    *p = __float2half_rn(v * scale);
+  } else {
+    (void)scale;
+    __builtin_nontemporal_store(v, p);
+  }
+}
+
+// Neighbour coordinates c-1 and c+1 along an axis of n nodes.  Off a
+// non-periodic edge the flag is cleared and the coordinate is left at c, so
+// every address formed from it stays inside the buffer.
+__device__ __forceinline__ void nbr(int c, int n, int per, int& cm, bool& vm, int& cp,
+                                    bool& vp) {
+  cm = c - 1;
+  vm = true;
+  cp = c + 1;
+  vp = true;
+  if (cm < 0) {
+    if (per) {
+      cm += n;
+    } else {
+      vm = false;
+      cm = c;
+    }
+  }
+  if (cp >= n) {
+    if (per) {
+      cp -= n;
+    } else {
+      vp = false;
+      cp = c;
+    }
+  }
+}
+
+// One wavefront streams 64 consecutive nodes of one row of the stored box:
+// a z-row (i,j) in 3D, a y-row i in 2D.  Everything that depends only on the
+// row is wave-uniform (scalar registers); a lane carries its column and
+// 32-bit byte offsets, so every access is a uniform 64-bit base plus a
+// 32-bit lane offset.
+struct Row {
+  int i, j;        // row coordinates (j unused in 2D)
+  int im, ip, jm, jp;
+  unsigned okmask; // bit 3a+b: neighbour row (i+a-1, j+b-1) exists
+  int ncols;
+  int c0;          // first column of this wave
+};
+
+template <int DIM>
+__device__ __forceinline__ bool make_row(const Geom& g, int64_t wave, Row& r) {
+  bool vim, vip, vjm = true, vjp = true;
+  const int ncols = DIM == 3 ? g.sz : g.sy;
+  const int nch = (ncols + 63) >> 6;
+  const int64_t row = wave / nch;
+  r.c0 = (int)(wave - row * nch) * 64;
+  r.ncols = ncols;
+  if constexpr (DIM == 3) {
+    if (row >= (int64_t)g.sx * g.sy) return false;
+    r.i = (int)(row / g.sy);
+    r.j = (int)(row - (int64_t)r.i * g.sy);
+    nbr(r.i, g.sx, g.per[0], r.im, vim, r.ip, vip);
+    nbr(r.j, g.sy, g.per[1], r.jm, vjm, r.jp, vjp);
+  } else {
+    if (row >= g.sx) return false;
+    r.i = (int)row;
+    r.j = 0;
+    r.jm = r.jp = 0;
+    nbr(r.i, g.sx, g.per[0], r.im, vim, r.ip, vip);
+  }
+  const bool iv[3] = {vim, true, vip}, jv[3] = {vjm, true, vjp};
+  unsigned m = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      if (iv[a] && jv[b]) m |= 1u << (3 * a + b);
+  r.okmask = m;
+  return true;
+}
+
+// Node index of column 0 of the row at offset (a-1, b-1) from row r.
+template <int DIM>
+__device__ __forceinline__ unsigned row_base(const Geom& g, const Row& r, int a, int b) {
+  const int ii = a == 0 ? r.im : (a == 2 ? r.ip : r.i);
+  if constexpr (DIM == 3) {
+    const int jj = b == 0 ? r.jm : (b == 2 ? r.jp : r.j);
+    return (unsigned)((ii * g.sy + jj) * g.sz);
+  } else {
+    return (unsigned)(ii * g.sy);
+  }
+}
+
+// Element p of plane q of a population buffer: uniform base, 32-bit offset.
+template <typename T>
+__device__ __forceinline__ const T* pq_ptr(const void* base, int q, int64_t plane, unsigned rowb) {
+  return (const T*)((const char*)base + ((size_t)q * (size_t)plane + rowb) * sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ T* pq_ptr_w(void* base, int q, int64_t plane, unsigned rowb) {
+  return (T*)((char*)base + ((size_t)q * (size_t)plane + rowb) * sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ T lane_ld(const T* p, unsigned col) {
+  return *(const T*)((const char*)p + col * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ T* lane_at(T* p, unsigned col) {
+  return (T*)((char*)p + col * (unsigned)sizeof(T));
+}
+
+// Pull the post-stream populations of the node in column c of row r, then
+// apply the Zou-He faces.  In post-stream mode the node's own values are
+// read and no face is applied (initial state / state set by the host).
+template <int DIM, int Q, int ST, bool FACES>
+__device__ __forceinline__ void gather_row(const CollideArgs& a, const Row& r, int c,
+                                           typename StoreT<ST>::C* f) {
+  using T = typename StoreT<ST>::T;
+  using C = typename StoreT<ST>::C;
+  const Geom& g = a.g;
+  const int64_t plane = g.plane;
+  if (a.poststream) {
+    const unsigned rb = row_base<DIM>(g, r, 1, 1);
+    Unroll<0, Q>::run([&]<int q>() {
+      f[q] = ld<ST>(lane_at(pq_ptr<T>(a.fin, q, plane, rb), (unsigned)c), a.inv_scale);
+    });
+    return;
+  }
+  const int per_c = DIM == 3 ? g.per[2] : g.per[1];
+  int cm, cp;
+  bool vcm, vcp;
+  nbr(c, r.ncols, per_c, cm, vcm, cp, vcp);
+  Unroll<0, Q>::run([&]<int q>() {
+    constexpr int ex = VelSet<Q>::ex[q], ey = VelSet<Q>::ey[q], ez = VelSet<Q>::ez[q];
+    // source node = node - e_q; the column axis is z in 3D and y in 2D
+    constexpr int ea = ex;
+    constexpr int eb = DIM == 3 ? ey : 0;
+    constexpr int ec = DIM == 3 ? ez : ey;
+    const unsigned col = (unsigned)(ec == 1 ? cm : (ec == -1 ? cp : c));
+    const bool cok = ec == 1 ? vcm : (ec == -1 ? vcp : true);
+    const bool rok = (r.okmask >> (3 * (1 - ea) + (1 - eb))) & 1u;
+    const T* rowp = pq_ptr<T>(a.fin, q, plane, row_base<DIM>(g, r, 1 - ea, 1 - eb));
+    f[q] = (rok && cok) ? ld<ST>(lane_at(rowp, col), a.inv_scale) : (C)a.cb[q];
+  });
+  if constexpr (FACES) {
+    const C r0 = (ST == ST_F64) ? C(0) : (C)a.rho0;
+    const int jj = DIM == 3 ? r.j : c;
+    const int kk = DIM == 3 ? c : 0;
+    apply_faces<Q, C>(f, a.faces, a.nfaces, r.i, jj, kk, r0);
+  }
+}
+
+// True when node (i,j,k) lies on one of the faces (plane and ranges).
+__device__ __forceinline__ bool on_face(const Face* faces, int nfaces, int i, int j, int k,
+                                        int first_face_only_before) {
+  for (int fi = 0; fi < nfaces && fi < first_face_only_before; ++fi) {
+    const Face& F = faces[fi];
+    const int c = F.axis == 0 ? i : (F.axis == 1 ? j : k);
+    if (c != F.plane) continue;
+    const int aa = F.axis == 0 ? j : i;
+    const int bb = F.axis == 2 ? j : k;
+    if (aa < F.a_lo || aa > F.a_hi || bb < F.b_lo || bb > F.b_hi) continue;
+    return true;
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 collision, reference expression order
+// (src/LBM_PSM_BGK_dynamics.h:40-114, src/LBM_PSM_dynamics.h:43-116).
+// ---------------------------------------------------------------------------
+// Collision of one node whose post-stream populations are in f[] (FP64).
+template <int DIM, int Q>
+__device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, unsigned rb,
+                                            unsigned uc);
+
+// Bulk: every node that is on no Zou-He face.
+template <int DIM, int Q>
+__global__ __launch_bounds__(64) void k_collide_f64(const CollideArgs a) {
+  const Geom& g = a.g;
+  Row r;
+  if (!make_row<DIM>(g, blockIdx.x, r)) return;
+  const int c = r.c0 + (int)threadIdx.x;
+  if (c >= r.ncols) return;
+  if (!a.poststream && a.nfaces &&
+      on_face(a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c, DIM == 3 ? c : 0, kMaxFaces))
+    return;
+  double f[Q];
+  gather_row<DIM, Q, ST_F64, false>(a, r, c, f);
+  collide_f64<DIM, Q>(a, f, row_base<DIM>(g, r, 1, 1), (unsigned)c);
+}
+
+template <int DIM, int Q>
+__device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, unsigned rb,
+                                            unsigned uc) {
+  const Geom& g = a.g;
+  const int64_t plane = g.plane;
+  const double omega = a.omega, inv2 = a.invCs2, inv4 = a.invCs4;
+  const double F0 = a.F[0], F1 = a.F[1], F2 = a.F[2];
+  {
+    const unsigned n = rb + uc;
+    double rho, jx, jy, jz;
+    moments<Q, double>(f, rho, jx, jy, jz);
+    jx = jx + F0 * 0.5;
+    jy = jy + F1 * 0.5;
+    jz = jz + F2 * 0.5;
+    const double ux = jx / rho, uy = jy / rho, uz = jz / rho;
+    if (a.rho_out) {
+      a.rho_out[n] = rho;
+      a.u_out[n] = ux;
+      a.u_out[plane + n] = uy;
+      a.u_out[2 * plane + n] = uz;
+    }
+    const double usqt = 0.5 * (ux * ux + uy * uy + uz * uz) * inv2;
+    const uint8_t m = a.mask[n];
+
+    if (m == 0) {
+      Unroll<0, Q>::run([&]<int q>() {
+        constexpr double w = VelSet<Q>::w[q];
+        const double eu = edot<Q, q>(ux, uy, uz);
+        const double feq = rho * w * (1.0 + eu * inv2 + eu * eu * 0.5 * inv4 - usqt);
+        double out = f[q] + (feq - f[q]) * omega;
+        if (a.guo) {
+          constexpr double ex = VelSet<Q>::ex[q], ey = VelSet<Q>::ey[q], ez = VelSet<Q>::ez[q];
+          const double eF = edot<Q, q>(F0, F1, F2);
+          const double Fi =
+              w * (((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) * inv2 + (eu * inv4 * eF));
+          out = out + a.guo_pref * Fi;
+        }
+        st<ST_F64>(lane_at(pq_ptr_w<double>(a.fout, q, plane, rb), uc), out, 1.0f);
+      });
+    } else {
+      // Occupied slots: B = solid fraction, u_s = rigid-body node velocity.
+      const bool o1 = m & 1, o2 = m & 2;
+      double B1 = o1 ? a.sf[n] : 0.0;
+      double B2 = o2 ? a.sf[plane + n] : 0.0;
+      double us1[3] = {0.0, 0.0, 0.0}, us2[3] = {0.0, 0.0, 0.0};
+      double Fa[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      if (o1) {
+        us1[0] = a.us[n];
+        us1[1] = a.us[plane + n];
+        us1[2] = a.us[2 * plane + n];
+        Fa[0] = a.Fh[n];
+        Fa[1] = a.Fh[plane + n];
+        Fa[2] = a.Fh[2 * plane + n];
+      }
+      if (o2) {
+        us2[0] = a.us[3 * plane + n];
+        us2[1] = a.us[4 * plane + n];
+        us2[2] = a.us[5 * plane + n];
+        Fa[3] = a.Fh[3 * plane + n];
+        Fa[4] = a.Fh[4 * plane + n];
+        Fa[5] = a.Fh[5 * plane + n];
+      }
+      const double u1sq = 0.5 * (us1[0] * us1[0] + us1[1] * us1[1] + us1[2] * us1[2]) * inv2;
+      const double u2sq = 0.5 * (us2[0] * us2[0] + us2[1] * us2[1] + us2[2] * us2[2]) * inv2;
+      const double Btot0 = B1 + B2;
+      double Btot = Btot0;
+      if (Btot0 > 1.0) {
+        B1 = B1 / Btot0;
+        B2 = B2 / Btot0;
+        Btot = 1.0;
+      }
+      const double oneMinusOmega = 1.0 - omega;
+      Unroll<0, Q>::run([&]<int q>() {
+        constexpr double w = VelSet<Q>::w[q];
+        constexpr int cx = VelSet<Q>::ex[q], cy = VelSet<Q>::ey[q], cz = VelSet<Q>::ez[q];
+        const double fi = f[q];
+        const double eu = edot<Q, q>(ux, uy, uz);
+        const double feq = rho * w * (1.0 + eu * inv2 + eu * eu * 0.5 * inv4 - usqt);
+        double sc1 = 0.0, sc2 = 0.0;
+        if (o1) {
+          const double e1 = edot<Q, q>(us1[0], us1[1], us1[2]);
+          const double fs = rho * w * (1.0 + e1 * inv2 + 0.5 * e1 * e1 * inv4 - u1sq);
+          sc1 = fs - fi + oneMinusOmega * (fi - feq);
+        }
+        if (o2) {
+          const double e2 = edot<Q, q>(us2[0], us2[1], us2[2]);
+          const double fs = rho * w * (1.0 + e2 * inv2 + 0.5 * e2 * e2 * inv4 - u2sq);
+          sc2 = fs - fi + oneMinusOmega * (fi - feq);
+        }
+        double out = fi + (1.0 - Btot) * (feq - fi) * omega + B1 * sc1 + B2 * sc2;
+        if (a.guo) {
+          constexpr double ex = cx, ey = cy, ez = cz;
+          const double eF = edot<Q, q>(F0, F1, F2);
+          const double Fi =
+              w * (((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) * inv2 + (eu * inv4 * eF));
+          out = out + a.guo_pref * Fi;
+        }
+        st<ST_F64>(lane_at(pq_ptr_w<double>(a.fout, q, plane, rb), uc), out, 1.0f);
+        // Node force on each slot, src/LBM_PSM_BGK_dynamics.h:101-113.
+        if (o1) {
+          const double t = -B1 * sc1;
+          if constexpr (cx > 0) Fa[0] = Fa[0] + t;
+          if constexpr (cx < 0) Fa[0] = Fa[0] - t;
+          if constexpr (cy > 0) Fa[1] = Fa[1] + t;
+          if constexpr (cy < 0) Fa[1] = Fa[1] - t;
+          if constexpr (cz > 0) Fa[2] = Fa[2] + t;
+          if constexpr (cz < 0) Fa[2] = Fa[2] - t;
+        }
+        if (o2) {
+          const double t = -B2 * sc2;
+          if constexpr (cx > 0) Fa[3] = Fa[3] + t;
+          if constexpr (cx < 0) Fa[3] = Fa[3] - t;
+          if constexpr (cy > 0) Fa[4] = Fa[4] + t;
+          if constexpr (cy < 0) Fa[4] = Fa[4] - t;
+          if constexpr (cz > 0) Fa[5] = Fa[5] + t;
+          if constexpr (cz < 0) Fa[5] = Fa[5] - t;
+        }
+      });
+      if (o1) {
+        a.Fh[n] = Fa[0];
+        a.Fh[plane + n] = Fa[1];
+        a.Fh[2 * plane + n] = Fa[2];
+      }
+      if (o2) {
+        a.Fh[3 * plane + n] = Fa[3];
+        a.Fh[4 * plane + n] = Fa[4];
+        a.Fh[5 * plane + n] = Fa[5];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 arithmetic on shifted populations g = f - w*rho0 (F32 / F16 storage).
+// ---------------------------------------------------------------------------
+template <int DIM, int Q, int ST>
+__device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, unsigned rb,
+                                             unsigned uc);
+
+template <int DIM, int Q, int ST>
+__global__ __launch_bounds__(64) void k_collide_lowp(const CollideArgs a) {
+  const Geom& g = a.g;
+  Row r;
+  if (!make_row<DIM>(g, blockIdx.x, r)) return;
+  const int c = r.c0 + (int)threadIdx.x;
+  if (c >= r.ncols) return;
+  if (!a.poststream && a.nfaces &&
+      on_face(a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c, DIM == 3 ? c : 0, kMaxFaces))
+    return;
+  float f[Q];
+  gather_row<DIM, Q, ST, false>(a, r, c, f);
+  collide_lowp<DIM, Q, ST>(a, f, row_base<DIM>(g, r, 1, 1), (unsigned)c);
+}
+
+template <int DIM, int Q, int ST>
+__device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, unsigned rb,
+                                             unsigned uc) {
+#pragma clang fp contract(fast)
+  const Geom& g = a.g;
+  const int64_t plane = g.plane;
+  using T = typename StoreT<ST>::T;
+  const float omega = (float)a.omega;
+  const float rho0 = (float)a.rho0;
+  const float F0 = (float)a.F[0], F1 = (float)a.F[1], F2 = (float)a.F[2];
+  const float gp = (float)a.guo_pref;
+  {
+    const unsigned n = rb + uc;
+    float drho, jx, jy, jz;
+    moments<Q, float>(f, drho, jx, jy, jz);
+    const float rho = rho0 + drho;
+    const float ir = 1.0f / rho;
+    const float ux = (jx + 0.5f * F0) * ir, uy = (jy + 0.5f * F1) * ir,
+                uz = (jz + 0.5f * F2) * ir;
+    if (a.rho_out) {
+      a.rho_out[n] = (double)rho0 + (double)drho;
+      a.u_out[n] = ux;
+      a.u_out[plane + n] = uy;
+      a.u_out[2 * plane + n] = uz;
+    }
+    const float usq = 1.5f * (ux * ux + uy * uy + uz * uz);
+    const uint8_t m = a.mask[n];
+    // g_eq = w (drho + rho (3 e.u + 4.5 (e.u)^2 - 1.5 u^2))
+    if (m == 0) {
+      Unroll<0, Q>::run([&]<int q>() {
+        constexpr float w = (float)VelSet<Q>::w[q];
+        const float eu = edot<Q, q>(ux, uy, uz);
+        const float geq = w * (drho + rho * (3.0f * eu + 4.5f * eu * eu - usq));
+        float out = f[q] + omega * (geq - f[q]);
+        if (a.guo) {
+          constexpr float ex = VelSet<Q>::ex[q], ey = VelSet<Q>::ey[q], ez = VelSet<Q>::ez[q];
+          const float eF = edot<Q, q>(F0, F1, F2);
+          out += gp * w * (3.0f * ((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) + 9.0f * eu * eF);
+        }
+        st<ST>(lane_at(pq_ptr_w<T>(a.fout, q, plane, rb), uc), out, a.scale);
+      });
+    } else {
+      const bool o1 = m & 1, o2 = m & 2;
+      float B1 = o1 ? (float)a.sf[n] : 0.0f;
+      float B2 = o2 ? (float)a.sf[plane + n] : 0.0f;
+      float us1[3] = {0.f, 0.f, 0.f}, us2[3] = {0.f, 0.f, 0.f};
+      float Fa[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      if (o1) {
+        us1[0] = (float)a.us[n];
+        us1[1] = (float)a.us[plane + n];
+        us1[2] = (float)a.us[2 * plane + n];
+      }
+      if (o2) {
+        us2[0] = (float)a.us[3 * plane + n];
+        us2[1] = (float)a.us[4 * plane + n];
+        us2[2] = (float)a.us[5 * plane + n];
+      }
+      const float u1sq = 1.5f * (us1[0] * us1[0] + us1[1] * us1[1] + us1[2] * us1[2]);
+      const float u2sq = 1.5f * (us2[0] * us2[0] + us2[1] * us2[1] + us2[2] * us2[2]);
+      float Btot = B1 + B2;
+      if (Btot > 1.0f) {
+        B1 = B1 / Btot;
+        B2 = B2 / Btot;
+        Btot = 1.0f;
+      }
+      const float omo = 1.0f - omega;
+      Unroll<0, Q>::run([&]<int q>() {
+        constexpr float w = (float)VelSet<Q>::w[q];
+        constexpr int cx = VelSet<Q>::ex[q], cy = VelSet<Q>::ey[q], cz = VelSet<Q>::ez[q];
+        const float gi = f[q];
+        const float eu = edot<Q, q>(ux, uy, uz);
+        const float geq = w * (drho + rho * (3.0f * eu + 4.5f * eu * eu - usq));
+        float sc1 = 0.f, sc2 = 0.f;
+        if (o1) {
+          const float e1 = edot<Q, q>(us1[0], us1[1], us1[2]);
+          const float gs = w * (drho + rho * (3.0f * e1 + 4.5f * e1 * e1 - u1sq));
+          sc1 = (gs - gi) + omo * (gi - geq);
+        }
+        if (o2) {
+          const float e2 = edot<Q, q>(us2[0], us2[1], us2[2]);
+          const float gs = w * (drho + rho * (3.0f * e2 + 4.5f * e2 * e2 - u2sq));
+          sc2 = (gs - gi) + omo * (gi - geq);
+        }
+        float out = gi + (1.0f - Btot) * omega * (geq - gi) + B1 * sc1 + B2 * sc2;
+        if (a.guo) {
+          constexpr float ex = cx, ey = cy, ez = cz;
+          const float eF = edot<Q, q>(F0, F1, F2);
+          out += gp * w * (3.0f * ((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) + 9.0f * eu * eF);
+        }
+        st<ST>(lane_at(pq_ptr_w<T>(a.fout, q, plane, rb), uc), out, a.scale);
+        const float t1 = -B1 * sc1, t2 = -B2 * sc2;
+        if constexpr (cx != 0) { Fa[0] += cx * t1; Fa[3] += cx * t2; }
+        if constexpr (cy != 0) { Fa[1] += cy * t1; Fa[4] += cy * t2; }
+        if constexpr (cz != 0) { Fa[2] += cz * t1; Fa[5] += cz * t2; }
+      });
+      if (o1) {
+        a.Fh[n] = a.Fh[n] + (double)Fa[0];
+        a.Fh[plane + n] = a.Fh[plane + n] + (double)Fa[1];
+        a.Fh[2 * plane + n] = a.Fh[2 * plane + n] + (double)Fa[2];
+      }
+      if (o2) {
+        a.Fh[3 * plane + n] = a.Fh[3 * plane + n] + (double)Fa[3];
+        a.Fh[4 * plane + n] = a.Fh[4 * plane + n] + (double)Fa[4];
+        a.Fh[5 * plane + n] = a.Fh[5 * plane + n] + (double)Fa[5];
+      }
+    }
+  }
+}
+
+// Generic per-node pull used by the read-back kernels (not on the hot path).
+template <int DIM, int Q, int ST>
+__device__ __forceinline__ void gather(const CollideArgs& a, int i, int j, int k,
+                                       typename StoreT<ST>::C* f) {
+  Row r;
+  const int64_t row = DIM == 3 ? (int64_t)i * a.g.sy + j : (int64_t)i;
+  const int ncols = DIM == 3 ? a.g.sz : a.g.sy;
+  const int c = DIM == 3 ? k : j;
+  make_row<DIM>(a.g, row * ((ncols + 63) / 64) + c / 64, r);
+  gather_row<DIM, Q, ST, true>(a, r, c, f);
+}
+
+// Boundary: nodes on at least one face.  Thread t walks the rectangle of
+// face `fi`; a node shared with an earlier face is left to that face's
+// threads, so each boundary node is updated once, with all its faces
+// applied in table order.
+template <int DIM, int Q, int ST>
+__global__ __launch_bounds__(64) void k_collide_faces(const CollideArgs a, int fi) {
+  const Face F = a.faces[fi];
+  const int na = F.a_hi - F.a_lo + 1, nb = F.b_hi - F.b_lo + 1;
+  const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (na <= 0 || nb <= 0 || t >= (int64_t)na * nb) return;
+  const int ta = F.a_lo + (int)(t / nb), tb = F.b_lo + (int)(t % nb);
+  int i, j, k;
+  if (F.axis == 0) { i = F.plane; j = ta; k = tb; }
+  else if (F.axis == 1) { i = ta; j = F.plane; k = tb; }
+  else { i = ta; j = tb; k = F.plane; }
+  const Geom& g = a.g;
+  if (i < 0 || i >= g.sx || j < 0 || j >= g.sy || k < 0 || k >= g.sz) return;
+  if (on_face(a.faces, a.nfaces, i, j, k, fi)) return;
+  typename StoreT<ST>::C f[Q];
+  gather<DIM, Q, ST>(a, i, j, k, f);
+  const unsigned rb = DIM == 3 ? (unsigned)((i * g.sy + j) * g.sz) : (unsigned)(i * g.sy);
+  const unsigned uc = DIM == 3 ? (unsigned)k : (unsigned)j;
+  if constexpr (ST == ST_F64) collide_f64<DIM, Q>(a, f, rb, uc);
+  else collide_lowp<DIM, Q, ST>(a, f, rb, uc);
+}
+
+// ---------------------------------------------------------------------------
+// Read-back helpers.
+// ---------------------------------------------------------------------------
+// Global interior coordinate of a stored node, or false for halo nodes.
+__device__ __forceinline__ bool interior(const Geom& g, int i, int j, int k, int& gi, int& gj,
+                                         int& gk) {
+  gi = i + g.off[0] - 1;
+  gj = j + g.off[1] - 1;
+  gk = (g.dim == 3) ? k + g.off[2] - 1 : 0;
+  return gi >= 0 && gi < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz;
+}
+
+template <int DIM, int Q, int ST>
+__global__ __launch_bounds__(256) void k_finish(const CollideArgs a, double* out) {
+  const Geom& g = a.g;
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= g.nodes) return;
+  const int k = (int)(n % g.sz);
+  const int j = (int)((n / g.sz) % g.sy);
+  const int i = (int)(n / ((int64_t)g.sz * g.sy));
+  int gi, gj, gk;
+  if (!interior(g, i, j, k, gi, gj, gk)) return;
+  typename StoreT<ST>::C f[Q];
+  gather<DIM, Q, ST>(a, i, j, k, f);
+  const int64_t o = (((int64_t)gi * g.ny + gj) * g.nz + gk) * Q;
+  Unroll<0, Q>::run([&]<int q>() {
+    if constexpr (ST == ST_F64) out[o + q] = f[q];
+    else out[o + q] = (double)f[q] + VelSet<Q>::w[q] * a.rho0;
+  });
+}
+
+template <int DIM, int Q, int ST>
+__global__ __launch_bounds__(256) void k_mass(const CollideArgs a, double* partials) {
+  const Geom& g = a.g;
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < g.nodes;
+       n += (int64_t)gridDim.x * 256) {
+    const int k = (int)(n % g.sz);
+    const int j = (int)((n / g.sz) % g.sy);
+    const int i = (int)(n / ((int64_t)g.sz * g.sy));
+    int gi, gj, gk;
+    if (!interior(g, i, j, k, gi, gj, gk)) continue;
+    typename StoreT<ST>::C f[Q];
+    gather<DIM, Q, ST>(a, i, j, k, f);
+    double s = 0.0;
+    Unroll<0, Q>::run([&]<int q>() { s += (double)f[q]; });
+    if constexpr (ST != ST_F64) s += a.rho0;
+    acc += s;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// Fill population plane q with one value given in storage form.
+template <int ST>
+__global__ __launch_bounds__(256) void k_fill(typename StoreT<ST>::T* f, int64_t plane, int q,
+                                              double v, float scale) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= plane) return;
+  if constexpr (ST == ST_F64) f[q * plane + n] = v;
+  else if constexpr (ST == ST_F32) f[q * plane + n] = (float)v;
+  else f[q * plane + n] = __float2half_rn((float)v * scale);
+}
+
+template <int Q, int ST>
+__global__ __launch_bounds__(256) void k_set_pop(typename StoreT<ST>::T* f, Geom g,
+                                                 const double* in, double rho0, float scale) {
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= g.nodes) return;
+  const int k = (int)(n % g.sz);
+  const int j = (int)((n / g.sz) % g.sy);
+  const int i = (int)(n / ((int64_t)g.sz * g.sy));
+  int gi, gj, gk;
+  if (!interior(g, i, j, k, gi, gj, gk)) return;
+  const int64_t o = (((int64_t)gi * g.ny + gj) * g.nz + gk) * Q;
+  Unroll<0, Q>::run([&]<int q>() {
+    const double v = in[o + q];
+    if constexpr (ST == ST_F64) f[q * g.plane + n] = v;
+    else if constexpr (ST == ST_F32) f[q * g.plane + n] = (float)(v - VelSet<Q>::w[q] * rho0);
+    else f[q * g.plane + n] = __float2half_rn((float)(v - VelSet<Q>::w[q] * rho0) * scale);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------
+static inline int64_t rows_of(const Geom& g) {
+  return g.dim == 3 ? (int64_t)g.sx * g.sy : (int64_t)g.sx;
+}
+
+template <int DIM, int Q>
+static hipError_t collide_dim(int storage, const CollideArgs& a, hipStream_t s) {
+  const int64_t rows = rows_of(a.g);
+  const int ncols = DIM == 3 ? a.g.sz : a.g.sy;
+  dim3 grid((unsigned)(rows * ((ncols + 63) / 64)));
+  switch (storage) {
+    case ST_F64: hipLaunchKernelGGL((k_collide_f64<DIM, Q>), grid, dim3(64), 0, s, a); break;
+    case ST_F32: hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32>), grid, dim3(64), 0, s, a); break;
+    case ST_F16: hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16>), grid, dim3(64), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  if (a.poststream) return hipGetLastError();
+  for (int fi = 0; fi < a.nfaces; ++fi) {
+    const Face& F = a.host_faces[fi];
+    const int64_t cnt = (int64_t)(F.a_hi - F.a_lo + 1) * (F.b_hi - F.b_lo + 1);
+    if (cnt <= 0) continue;
+    dim3 fg((unsigned)((cnt + 63) / 64));
+    switch (storage) {
+      case ST_F64: hipLaunchKernelGGL((k_collide_faces<DIM, Q, ST_F64>), fg, dim3(64), 0, s, a, fi); break;
+      case ST_F32: hipLaunchKernelGGL((k_collide_faces<DIM, Q, ST_F32>), fg, dim3(64), 0, s, a, fi); break;
+      case ST_F16: hipLaunchKernelGGL((k_collide_faces<DIM, Q, ST_F16>), fg, dim3(64), 0, s, a, fi); break;
+      default: return hipErrorInvalidValue;
+    }
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_collide(int storage, const CollideArgs& a, hipStream_t s) {
+  return a.g.dim == 3 ? collide_dim<3, 19>(storage, a, s) : collide_dim<2, 9>(storage, a, s);
+}
+
+template <int DIM, int Q>
+static hipError_t finish_dim(int storage, const CollideArgs& a, double* out, hipStream_t s) {
+  dim3 grid((unsigned)((a.g.nodes + 255) / 256));
+  switch (storage) {
+    case ST_F64: hipLaunchKernelGGL((k_finish<DIM, Q, ST_F64>), grid, dim3(256), 0, s, a, out); break;
+    case ST_F32: hipLaunchKernelGGL((k_finish<DIM, Q, ST_F32>), grid, dim3(256), 0, s, a, out); break;
+    case ST_F16: hipLaunchKernelGGL((k_finish<DIM, Q, ST_F16>), grid, dim3(256), 0, s, a, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_finish(int storage, const CollideArgs& a, double* out, hipStream_t s) {
+  return a.g.dim == 3 ? finish_dim<3, 19>(storage, a, out, s) : finish_dim<2, 9>(storage, a, out, s);
+}
+
+template <int DIM, int Q>
+static hipError_t mass_dim(int storage, const CollideArgs& a, double* partials, int nb,
+                           hipStream_t s) {
+  switch (storage) {
+    case ST_F64: hipLaunchKernelGGL((k_mass<DIM, Q, ST_F64>), dim3(nb), dim3(256), 0, s, a, partials); break;
+    case ST_F32: hipLaunchKernelGGL((k_mass<DIM, Q, ST_F32>), dim3(nb), dim3(256), 0, s, a, partials); break;
+    case ST_F16: hipLaunchKernelGGL((k_mass<DIM, Q, ST_F16>), dim3(nb), dim3(256), 0, s, a, partials); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_mass(int storage, const CollideArgs& a, double* partials, int nb,
+                       hipStream_t s) {
+  return a.g.dim == 3 ? mass_dim<3, 19>(storage, a, partials, nb, s)
+                      : mass_dim<2, 9>(storage, a, partials, nb, s);
+}
+
+// cb holds one value per direction in storage form (raw for F64, shifted
+// for F32/F16).
+hipError_t launch_fill_planes(int storage, void* f, const Geom& g, const double* cb,
+                              double rho0, float scale, hipStream_t s) {
+  (void)rho0;
+  dim3 grid((unsigned)((g.plane + 255) / 256));
+  for (int q = 0; q < g.q; ++q) {
+    switch (storage) {
+      case ST_F64: hipLaunchKernelGGL((k_fill<ST_F64>), grid, dim3(256), 0, s, (double*)f, g.plane, q, cb[q], scale); break;
+      case ST_F32: hipLaunchKernelGGL((k_fill<ST_F32>), grid, dim3(256), 0, s, (float*)f, g.plane, q, cb[q], scale); break;
+      case ST_F16: hipLaunchKernelGGL((k_fill<ST_F16>), grid, dim3(256), 0, s, (__half*)f, g.plane, q, cb[q], scale); break;
+      default: return hipErrorInvalidValue;
+    }
+  }
+  return hipGetLastError();
+}
+
+template <int Q>
+static hipError_t setpop_q(int storage, void* f, const Geom& g, const double* in, double rho0,
+                           float scale, hipStream_t s) {
+  dim3 grid((unsigned)((g.nodes + 255) / 256));
+  switch (storage) {
+    case ST_F64: hipLaunchKernelGGL((k_set_pop<Q, ST_F64>), grid, dim3(256), 0, s, (double*)f, g, in, rho0, scale); break;
+    case ST_F32: hipLaunchKernelGGL((k_set_pop<Q, ST_F32>), grid, dim3(256), 0, s, (float*)f, g, in, rho0, scale); break;
+    case ST_F16: hipLaunchKernelGGL((k_set_pop<Q, ST_F16>), grid, dim3(256), 0, s, (__half*)f, g, in, rho0, scale); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_set_populations(int storage, void* f, const Geom& g, const double* in,
+                                  double rho0, float scale, hipStream_t s) {
+  return g.q == 19 ? setpop_q<19>(storage, f, g, in, rho0, scale, s)
+                   : setpop_q<9>(storage, f, g, in, rho0, scale, s);
+}
+
+}  // namespace lbmpsm

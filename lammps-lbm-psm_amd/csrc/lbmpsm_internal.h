@@ -1,0 +1,170 @@
+// Internal interfaces between the C ABI layer (api.hip) and the kernel
+// translation units.  Nothing here is exported.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lbm_common.h"
+
+namespace lbmpsm {
+
+constexpr int kMaxFaces = 6;
+
+// Stored lattice geometry.  Node (i,j,k) of the stored box sits at
+// n = (i*sy + j)*sz + k; each population is a plane of `plane` elements.
+// Periodic axes hold exactly the global nodes and wrap in the kernels; a
+// non-periodic axis keeps the reference's halo layer on both sides, so its
+// stored coordinate equals the reference's halo'd index.  `off` is the
+// reference index minus the stored index per axis.
+struct Geom {
+  int dim, q;
+  int sx, sy, sz;
+  int per[3];
+  int off[3];
+  int nx, ny, nz;  // global interior counts (nz = 1 in 2D)
+  int64_t nodes;   // sx*sy*sz
+  int64_t plane;   // nodes rounded up to a multiple of 64
+};
+
+// Pull-stream + BC + collide for one node.  fin/fout point at plane 0 of a
+// population buffer (element type depends on the storage form).
+struct CollideArgs {
+  Geom g;
+  const void* fin;
+  void* fout;
+  const uint8_t* mask;  // bit s set: slot s holds a particle (tag > 0)
+  const double* sf;     // [2][plane]
+  const double* us;     // [6][plane]   (3*s + d)
+  double* Fh;           // [6][plane]   (3*s + d), lattice units
+  double* rho_out;      // [plane]      or null
+  double* u_out;        // [3][plane]   or null
+  int poststream;       // 1: fin already holds post-stream values (no pull, no BC)
+  int guo;              // |F|^2 > 0
+  double omega;
+  double guo_pref;      // 1 - 0.5*omega
+  double F[3];
+  double invCs2, invCs4;
+  double rho0;          // reference density of shifted storage
+  float scale, inv_scale;  // FP16 storage scaling (power of two)
+  const double* cb;     // [q] value of populations pulled from beyond a non-periodic
+                        // halo, in storage form (device memory)
+  const Face* faces;    // [nfaces] Zou-He faces in application order (device memory)
+  int nfaces;
+  const Face* host_faces;  // the same table in host memory (launch sizes only)
+};
+
+// Particle list state on the device (SoA, capacity entries).
+struct ParticleDev {
+  int cap = 0;
+  int n = 0;
+  // host->device staging (raw LAMMPS data)
+  int* tag = nullptr;
+  double* x = nullptr;      // [n][3]
+  double* v = nullptr;      // [n][3]
+  double* w = nullptr;      // [n][3]
+  double* r = nullptr;      // [n]
+  // derived
+  double* xl = nullptr;     // [n][4]  x_loc, y_loc, z_loc (reference halo'd coords), r_lb
+  double* kin = nullptr;    // [n][6]  uTrans, omegaLB
+  int* zone = nullptr;      // [n][6]  stored-coordinate map zone [lo,hi) x,y,z
+  int* rzone = nullptr;     // [n][6]  stored-coordinate reduction zone
+  double* out = nullptr;    // [n][12] F, T, S (physical units)
+};
+
+struct BrickDev {
+  int bi = 0, bj = 0, bk = 0;  // brick edge lengths (nodes)
+  int nbi = 0, nbj = 0, nbk = 0;
+  int64_t nbricks = 0;
+  int* count = nullptr;     // [nbricks + 1]
+  int* offset = nullptr;    // [nbricks + 1]
+  int* cursor = nullptr;    // [nbricks]
+  int* nonempty = nullptr;  // [nbricks]
+  int* nonempty_n = nullptr;  // [1]
+  int* block_sums = nullptr;  // scan scratch
+  int* list = nullptr;      // [list_cap]
+  int64_t list_cap = 0;
+  int* overflow = nullptr;  // [1] lists longer than the map kernel's LDS
+};
+
+struct SlotsDev {
+  int* tag = nullptr;       // [2][plane]
+  double* sf = nullptr;     // [2][plane]
+  double* us = nullptr;     // [6][plane]
+  double* Fh = nullptr;     // [6][plane]
+  uint8_t* mask = nullptr;  // [plane]
+};
+
+struct MapArgs {
+  Geom g;
+  SlotsDev s;
+  const int* tag;
+  const double* xl;
+  const double* kin;
+  const int* zone;
+  const int* list;
+  const int* offset;
+  const int* count;
+  const int* nonempty;
+  const int* nonempty_n;
+  int bi, bj, bk, nbi, nbj, nbk;
+  double hsq_half;
+};
+
+constexpr int kMapMaxList = 2048;  // particles per brick the map kernel sorts in LDS
+
+// ---- launchers (lattice_f64.hip / lattice_lowp.hip) ------------------------
+hipError_t launch_collide(int storage, const CollideArgs& a, hipStream_t s);
+hipError_t launch_fill_planes(int storage, void* f, const Geom& g, const double* cb,
+                              double rho0, float scale, hipStream_t s);
+// Post-stream + BC values of interior nodes into `out` (FP64, reference
+// order [nx][ny][nz][q]).
+hipError_t launch_finish(int storage, const CollideArgs& a, double* out, hipStream_t s);
+// Interior reference-order FP64 populations (post-stream) into storage form.
+hipError_t launch_set_populations(int storage, void* f, const Geom& g, const double* in,
+                                  double rho0, float scale, hipStream_t s);
+// Mass of the post-stream state: per-block partial sums into `partials`.
+hipError_t launch_mass(int storage, const CollideArgs& a, double* partials, int nblocks,
+                       hipStream_t s);
+
+// ---- particles.hip ----------------------------------------------------------
+struct PrepArgs {
+  Geom g;
+  int n;
+  const double* x;
+  const double* v;
+  const double* w;
+  const double* r;
+  double* xl;
+  double* kin;
+  int* zone;
+  int* rzone;
+  double origin[3];
+  double dx, Uc, u_lb, tc, dt_d;
+  int* count;
+  int bi, bj, bk, nbi, nbj, nbk;
+};
+hipError_t launch_fill_int(int* p, int64_t n, int value, hipStream_t s);
+hipError_t launch_prep_count(const PrepArgs& a, hipStream_t s);
+hipError_t launch_scan_bricks(BrickDev& b, hipStream_t s);
+hipError_t launch_fill_lists(const PrepArgs& a, const BrickDev& b, hipStream_t s);
+hipError_t launch_map(const MapArgs& a, int* overflow, hipStream_t s, int grid);
+struct HydroArgs {
+  Geom g;
+  int n;
+  const int* tag;
+  const double* xl;
+  const int* rzone;
+  const int* slot_tag;
+  const double* Fh;
+  double forceFactor, torqueFactor;
+  double* out;
+};
+hipError_t launch_hydro(const HydroArgs& a, hipStream_t s);
+hipError_t launch_slots_export(const Geom& g, const SlotsDev& sl, int* tag, double* sf,
+                               double* us, double* F, hipStream_t s);
+hipError_t launch_macros_export(const Geom& g, const double* rho, const double* u,
+                                double* rho_out, double* u_out, hipStream_t s);
+hipError_t launch_slots_reset(const Geom& g, const SlotsDev& sl, hipStream_t s);
+
+}  // namespace lbmpsm

@@ -1,0 +1,204 @@
+"""GPU library vs CPU oracle on identical mock-LAMMPS inputs (FP64 path).
+
+Bar: particle-slot state (tags, solid fractions) bit-exact; populations,
+rho/u and slot velocities/forces equal to the oracle within 1e-12 of the
+largest magnitude (in practice bit-exact: the kernels keep the reference's
+operand order); per-entry force/torque/stresslet within 1e-12 of the sum of
+absolute terms (the reductions run in a different order).
+"""
+import numpy as np
+import pytest
+
+import lbmpsm
+import lbmpsm_oracle as O
+from lbmpsm import harness as H
+from scenarios import BC_IDS, hydro_scales, lattice_box, random_atoms, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def run_pair(dim, n, periodic, tau, F, bc, atoms, steps, cutoff=None, move=True,
+             store_macros=True, label=""):
+    print(f"\n[case] {label} dim={dim} n={n} periodic={periodic} bc={bc} steps={steps}", flush=True)
+    box = lattice_box(dim, n, periodic)
+    lat = lbmpsm.Lattice(lbmpsm.lattice_units_config(dim, n, periodic, tau, F_lbm=F))
+    ora = O.Oracle(dim, n, periodic, tau, F_lbm=F)
+    if bc != "none":
+        lat.set_bc(bc)
+    out = []
+    for step in range(steps):
+        lst = None
+        if atoms is not None:
+            if move:
+                H.move_linear(atoms, box, step, 1.0)
+            cut = cutoff if cutoff is not None else float(np.max(atoms.radius)) + 4.0
+            lst = H.build_list(atoms, box, cut)
+        else:
+            lst = H.AtomList(tag=np.zeros(0, np.int32), x=np.zeros((0, 3)), v=np.zeros((0, 3)),
+                             omega=np.zeros((0, 3)), radius=np.zeros(0), owner=np.zeros(0, int),
+                             nlocal=0)
+        Fo, To, So = ora.fix_step(lst, bc_keyword=BC_IDS[bc])
+        lat.set_particles(lst.tag, lst.x, lst.v, lst.omega, lst.radius)
+        lat.step(store_macros=store_macros)
+        Fg, Tg, Sg = lat.hydro()
+        out.append((lst, (Fg, Tg, Sg), (Fo, To, So)))
+    return lat, ora, out
+
+
+def compare(lat, ora, out, check_macros=True):
+    fg, fo = lat.populations(), ora.populations()
+    assert fg.shape == fo.shape
+    e_pop = rel_err(fg, fo)
+    tg, sfg, usg, Fhg = lat.slots()
+    to, sfo, uso, Fho = ora.slots()
+    assert np.array_equal(tg, to), "slot tags differ"
+    assert np.array_equal(sfg, sfo), "solid fractions differ"
+    e_us = rel_err(usg, uso) if np.any(uso) else 0.0
+    e_Fh = rel_err(Fhg, Fho) if np.any(Fho) else 0.0
+    print(f"  pop rel {e_pop:.2e}  slot-u rel {e_us:.2e}  node-F rel {e_Fh:.2e}", flush=True)
+    assert e_pop <= TOL
+    assert e_us <= TOL
+    assert e_Fh <= TOL
+    if check_macros:
+        rg, ug = lat.macros()
+        ro, uo = ora.macros()
+        assert rel_err(rg, ro) <= TOL
+        assert rel_err(ug, uo) <= TOL or np.max(np.abs(ug - uo)) <= 1e-15
+    lst, (Fg, Tg, Sg), (Fo, To, So) = out[-1]
+    if lst.n:
+        sF, sT = hydro_scales(lst, to, Fho)
+        dF = np.max(np.abs(Fg - Fo), axis=1)
+        dT = np.max(np.abs(Tg - To), axis=1)
+        dS = np.max(np.abs(Sg - So), axis=1)
+        print(f"  F/T/S max diff / scale: {np.max(dF / (sF + 1e-300)):.2e} "
+              f"{np.max(dT / (sT + 1e-300)):.2e} {np.max(dS / (sT + 1e-300)):.2e}", flush=True)
+        assert np.all(dF <= TOL * sF + 1e-300)
+        assert np.all(dT <= TOL * sT + 1e-300)
+        assert np.all(dS <= TOL * sT + 1e-300)
+
+
+def test_fluid_periodic_guo_3d():
+    lat, ora, out = run_pair(3, (20, 18, 16), (1, 1, 1), 0.7, (1e-5, -2e-6, 3e-6), "none", None,
+                             12, label="fluid")
+    compare(lat, ora, out)
+    fg, fo = lat.populations(), ora.populations()
+    assert np.array_equal(fg, fo), "FP64 fluid populations are not bit-exact"
+
+
+def test_fluid_periodic_guo_2d():
+    lat, ora, out = run_pair(2, (32, 24, 1), (1, 1, 1), 0.65, (1e-5, 2e-6, 0.0), "none", None,
+                             15, label="fluid2d")
+    compare(lat, ora, out)
+    assert np.array_equal(lat.populations(), ora.populations())
+
+
+def test_spheres_shear_3d_moving():
+    n = (28, 26, 24)
+    per = (1, 0, 1)
+    box = lattice_box(3, n, per)
+    atoms = random_atoms(box, [4.0, 3.5, 4.5, 3.0], seed=3, vmax=2e-2, wmax=3e-3, gap=0.2)
+    lat, ora, out = run_pair(3, n, per, 0.8, (0.0, 0.0, 0.0), "shear", atoms, 10, label="shear")
+    compare(lat, ora, out)
+
+
+def test_spheres_periodic_guo_shared_nodes():
+    # two spheres nearly touching (shared rim nodes) and one crossing x = 0
+    n = (24, 24, 24)
+    per = (1, 1, 1)
+    pos = np.array([[10.0, 12.2, 11.9], [17.6, 12.4, 12.1], [0.7, 5.3, 18.2]])
+    box = lattice_box(3, n, per)
+    atoms = random_atoms(box, [3.7, 3.6, 3.2], seed=5, positions=pos, vmax=1e-2, wmax=1e-3)
+    lat, ora, out = run_pair(3, n, per, 0.65, (2e-6, 0.0, 0.0), "none", atoms, 8, label="shared")
+    compare(lat, ora, out)
+    tag, sf, _, _ = lat.slots()
+    assert np.any((tag[..., 0] > 0) & (tag[..., 1] > 0)), "scenario has no shared node"
+
+
+def test_three_particles_one_node_drop():
+    # three spheres whose rims meet near one node: the third is dropped there
+    n = (20, 20, 20)
+    per = (1, 1, 1)
+    c = np.array([10.0, 10.0, 10.0])
+    r = 3.0
+    d = r + 0.55
+    pos = np.array([c + [d, 0, 0], c + [-d / 2, d * 0.866, 0], c + [-d / 2, -d * 0.866, 0]])
+    box = lattice_box(3, n, per)
+    atoms = random_atoms(box, [r, r, r], seed=1, positions=pos, vmax=0.0, wmax=0.0)
+    lat, ora, out = run_pair(3, n, per, 0.9, (0, 0, 0), "none", atoms, 3, move=False,
+                             label="drop")
+    compare(lat, ora, out)
+
+
+@pytest.mark.parametrize("bc", ["shear", "xFlow", "channel", "channel-velIn-pressOut",
+                                "closedBox"])
+@pytest.mark.parametrize("per", [(0, 0, 0), (1, 0, 1), (1, 1, 1), (0, 1, 1)])
+def test_bc_keywords_3d(bc, per):
+    n = (18, 16, 14)
+    box = lattice_box(3, n, per)
+    atoms = random_atoms(box, [3.0, 2.5], seed=11, vmax=5e-3, wmax=5e-4, gap=1.5)
+    lat, ora, out = run_pair(3, n, per, 0.75, (1e-6, 0, 0), bc, atoms, 6,
+                             label=f"bc3d-{bc}-{per}")
+    compare(lat, ora, out)
+
+
+@pytest.mark.parametrize("bc", ["shear", "xFlow", "channel", "channel-velIn-pressOut",
+                                "closedBox"])
+@pytest.mark.parametrize("per", [(0, 0, 1), (1, 0, 1), (1, 1, 1)])
+def test_bc_keywords_2d(bc, per):
+    n = (30, 22, 1)
+    box = lattice_box(2, n, per)
+    atoms = random_atoms(box, [4.0, 3.0], seed=7, vmax=5e-3, wmax=5e-4, gap=1.5)
+    lat, ora, out = run_pair(2, n, per, 0.7, (1e-6, 0, 0), bc, atoms, 8, label=f"bc2d-{bc}-{per}")
+    compare(lat, ora, out)
+
+
+def test_disk_2d_fixed_guo():
+    # config-1 geometry at reduced size: disk d=16 at the centre, Fx = 1e-6
+    n = (64, 64, 1)
+    per = (1, 1, 1)
+    box = lattice_box(2, n, per)
+    atoms = random_atoms(box, [8.0], seed=0, positions=np.array([[32.0, 32.0, 0.0]]), vmax=0.0,
+                         wmax=0.0)
+    lat, ora, out = run_pair(2, n, per, 0.65, (1e-6, 0, 0), "none", atoms, 30, move=False,
+                             label="disk2d")
+    compare(lat, ora, out)
+
+
+def test_populations_roundtrip_and_mass():
+    n = (12, 10, 8)
+    per = (1, 0, 1)
+    print("\n[case] set/get populations, mass, continue 4 shear steps", flush=True)
+    lat = lbmpsm.Lattice(lbmpsm.lattice_units_config(3, n, per, 0.7))
+    ora = O.Oracle(3, n, per, 0.7)
+    rng = np.random.default_rng(2)
+    f = ora.populations() * (1.0 + 0.01 * rng.standard_normal((12, 10, 8, 19)))
+    lat.set_populations(f)
+    assert np.array_equal(lat.populations(), f)
+    assert abs(lat.mass() - f.sum()) <= 1e-12 * f.sum()
+    fh = ora.f_halo()
+    fh[1:-1, 1:-1, 1:-1] = f
+    ora.set_f_halo(fh)
+    lat.set_bc("shear")
+    empty = H.AtomList(tag=np.zeros(0, np.int32), x=np.zeros((0, 3)), v=np.zeros((0, 3)),
+                       omega=np.zeros((0, 3)), radius=np.zeros(0), owner=np.zeros(0, int),
+                       nlocal=0)
+    for _ in range(4):
+        ora.fix_step(empty, bc_keyword=1)
+        lat.set_particles(empty.tag, empty.x, empty.v, empty.omega, empty.radius)
+        lat.step()
+    assert rel_err(lat.populations(), ora.populations()) <= TOL
+
+
+def test_zero_particles_and_errors():
+    lat = lbmpsm.Lattice(lbmpsm.lattice_units_config(3, (8, 8, 8), (1, 1, 1), 0.7))
+    lat.set_particles(np.zeros(0, np.int32), np.zeros((0, 3)), np.zeros((0, 3)), np.zeros((0, 3)),
+                      np.zeros(0))
+    lat.step()
+    F, T, S = lat.hydro()
+    assert F.shape == (0, 3)
+    with pytest.raises(lbmpsm.LbmPsmError):
+        lat.set_bc(99, 0.0)
+    with pytest.raises(lbmpsm.LbmPsmError):
+        lbmpsm.Lattice(lbmpsm.lattice_units_config(3, (8, 8, 8), (1, 1, 1), 0.5))
