@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""LBM-PSM throughput on one MI355X: D3Q19 BGK-PSM, 192^3 nodes, shear walls,
+400 moving spheres (d = 16, seeded RSA, ~12 % solids).
+
+One step = one `fix lbm-psm` LBM step as the fix drives it: upload the atom
+list (locals + periodic ghosts) and map it onto the lattice, fused
+collide-stream with PSM weighting and the Zou-He shear faces, the
+per-particle force/torque/stresslet reduction, and the download of those
+results.  Prints one JSON line (contract in the repository instructions).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--storage f64|f32|f16]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lammps-lbm-psm_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+Q = 19
+NX = NY = NZ = 192
+TAU = 0.7            # `fix lbm-psm` default (src/fix_LBM_PSM.cpp:23); the workload names none
+U_WALL = 0.01
+N_SPHERES = 400
+RADIUS = 8.0
+SEED = 20240611
+METRIC = "MLUPS + % HBM roofline, D3Q19 PSM: FP64 vs FP32-storage at stated drag error"
+BYTES_PER_ELEM = {"f64": 8, "f32": 4, "f16": 2}
+
+
+def workload():
+    from lbmpsm import harness as H
+    # LAMMPS `boundary p f p`: 192 periodic nodes in x and z, 192 wall-to-wall
+    # nodes in y (Ly = 191 lattice units)
+    box = H.Box(lo=np.zeros(3), prd=np.array([192.0, 191.0, 192.0]), periodic=(1, 0, 1), dim=3)
+    radii = np.full(N_SPHERES, RADIUS)
+    pos = H.rsa_spheres(box, radii, SEED, gap=2.5)
+    rng = np.random.default_rng(SEED + 1)
+    v = rng.uniform(-1e-3, 1e-3, (N_SPHERES, 3))
+    w = rng.uniform(-1e-4, 1e-4, (N_SPHERES, 3))
+    atoms = H.Atoms(tag=np.arange(1, N_SPHERES + 1, dtype=np.int32), x=pos, v=v, omega=w,
+                    radius=radii)
+    return box, atoms
+
+
+def atom_lists(box, atoms, nsteps):
+    from lbmpsm import harness as H
+    a = atoms.copy()
+    out = []
+    for s in range(nsteps):
+        H.move_linear(a, box, s, 1.0)
+        out.append(H.build_list(a, box, cutoff=RADIUS + 4.0))
+    return out
+
+
+def run_gpu(storage, lists, warmup, steps):
+    import lbmpsm
+    cfg = lbmpsm.lattice_units_config(3, (NX, NY, NZ), (1, 0, 1), TAU, storage=storage,
+                                      u_lb=U_WALL)
+    lat = lbmpsm.Lattice(cfg)
+    lat.set_bc("shear")
+    F = None
+    for s in range(warmup):
+        L = lists[s]
+        lat.set_particles(L.tag, L.x, L.v, L.omega, L.radius)
+        lat.step()
+        F, T, S = lat.hydro()
+    lat.sync()
+    lat.profile(True)
+    lat.profile_reset()
+    t0 = time.perf_counter()
+    for s in range(warmup, warmup + steps):
+        L = lists[s]
+        lat.set_particles(L.tag, L.x, L.v, L.omega, L.radius)
+        lat.step()
+        F, T, S = lat.hydro()
+    lat.sync()
+    t1 = time.perf_counter()
+    prof = {k: lat.profile_read(k) for k in ("collide_stream", "collide_faces", "map_bin",
+                                              "map_particles", "hydro_reduce")}
+    lat.profile(False)
+    info = lat.info
+    lat.close()
+    return t1 - t0, F, T, S, prof, info
+
+
+def cpu_baseline(box, atoms, nsteps=3):
+    """The CPU oracle (reference algorithm and layout, OpenMP over x) on the
+    same workload: nsteps full LBM steps after one untimed step."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lbmpsm_oracle as O
+    lists = atom_lists(box, atoms, nsteps + 1)
+    ora = O.Oracle(3, (NX, NY, NZ), (1, 0, 1), TAU, unit=O.lattice_units(U_WALL))
+    ora.fix_step(lists[0], bc_keyword=1)
+    t0 = time.perf_counter()
+    for s in range(1, nsteps + 1):
+        ora.fix_step(lists[s], bc_keyword=1)
+    t1 = time.perf_counter()
+    ora.close()
+    mlups = NX * NY * NZ * nsteps / (t1 - t0) / 1e6
+    return {"value": round(mlups, 3), "unit": "MLUPS", "cores": O.threads(), "kind": "port",
+            "sample": f"{nsteps} full steps of the same 192^3 / 400-sphere workload "
+                      f"(after 1 untimed step), FP64, {O.threads()} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--storage", default="f64", choices=["f64", "f32", "f16"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the lattice would shard by slabs with a 5-population face exchange; "
+              "see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    box, atoms = workload()
+    lists = atom_lists(box, atoms, args.warmup + args.steps)
+    nodes = NX * NY * NZ
+
+    results = {}
+    order = [args.storage] + ([] if args.no_variants else
+                              [s for s in ("f64", "f32", "f16") if s != args.storage])
+    for st in order:
+        print(f"[bench] storage {st}: {args.warmup} warm-up + {args.steps} timed steps",
+              file=sys.stderr, flush=True)
+        results[st] = run_gpu(st, lists, args.warmup, args.steps)
+
+    el, F, T, S, prof, info = results[args.storage]
+    ms_step = el / args.steps * 1e3
+    mlups = nodes * args.steps / el / 1e6
+    # dominant kernel: the bulk collide-stream (every node not on a face)
+    bulk_nodes = info.stored_nodes - 2 * NX * NZ  # the two shear-face planes run separately
+    ms_c, n_c = prof["collide_stream"]
+    bytes_per_update = 2 * Q * BYTES_PER_ELEM[args.storage]
+    avg_ms = ms_c / max(n_c, 1)
+    achieved = bytes_per_update * bulk_nodes / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_collide_f64<3,19>" if args.storage == "f64" else
+                      f"k_collide_lowp<3,19,{ {'f32': 1, 'f16': 2}[args.storage]}>",
+            "bytes_per_update": bytes_per_update, "nodes_per_launch": int(bulk_nodes),
+            "avg_launch_ms": round(avg_ms, 5)}
+    kernels = {k: {"avg_ms": round(v[0] / max(v[1], 1), 5), "launches": v[1]}
+               for k, v in prof.items()}
+
+    variants = {}
+    Fref = results["f64"][1] if "f64" in results else None
+    for st, (el2, F2, T2, S2, prof2, _) in results.items():
+        d = {"MLUPS": round(nodes * args.steps / el2 / 1e6, 1),
+             "ms_per_step": round(el2 / args.steps * 1e3, 4)}
+        if Fref is not None and st != "f64":
+            scale = np.max(np.linalg.norm(Fref, axis=1))
+            d["drag_rel_err_vs_f64"] = float(np.max(np.linalg.norm(F2 - Fref, axis=1)) / scale)
+            tscale = np.max(np.linalg.norm(results["f64"][2], axis=1))
+            d["torque_rel_err_vs_f64"] = float(np.max(np.linalg.norm(T2 - results["f64"][2],
+                                                                     axis=1)) / tscale)
+        variants[st] = d
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(box, atoms, args.cpu_steps)
+
+    line = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.storage,
+        "data": "synthetic",
+        "config": {"workload": "D3Q19 BGK-PSM 192^3, shear (Zou-He walls y, U=0.01), "
+                               "400 moving spheres d=16 (seeded RSA), tau=0.7",
+                   "nodes": nodes, "spheres": N_SPHERES, "list_entries": int(lists[-1].n),
+                   "storage": args.storage},
+        "roofline": roof,
+        "kernels": kernels,
+        "variants": variants,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -52,6 +52,9 @@ struct lbmpsm_ctx {
 
   ParticleDev part;
   BrickDev br;
+  unsigned short* sfcode = nullptr;  // per-entry zone solid-fraction codes
+  int zone_cap = 0;                  // codes per entry
+  size_t sfcode_elems = 0;
   // pinned host staging for one particle upload and the hydro download
   void* h_stage = nullptr;
   size_t h_stage_bytes = 0;
@@ -260,14 +263,10 @@ int ensure_particles(lbmpsm_ctx* c, int n) {
   // earlier steps may still read these buffers
   HIPCHK(c, hipStreamSynchronize(c->stream));
   int cap = std::max(n, std::max(64, P.cap * 2));
-  void* old[] = {P.tag, P.x, P.v, P.w, P.r, P.xl, P.kin, P.zone, P.rzone, P.out};
+  void* old[] = {P.raw, P.xl, P.kin, P.zone, P.rzone, P.out};
   for (void* p : old) dev_free(p);
   int rc = 0;
-  rc |= dev_alloc(c, &P.tag, cap);
-  rc |= dev_alloc(c, &P.x, 3 * (size_t)cap);
-  rc |= dev_alloc(c, &P.v, 3 * (size_t)cap);
-  rc |= dev_alloc(c, &P.w, 3 * (size_t)cap);
-  rc |= dev_alloc(c, &P.r, cap);
+  rc |= dev_alloc(c, &P.raw, 84 * (size_t)cap);
   rc |= dev_alloc(c, &P.xl, 4 * (size_t)cap);
   rc |= dev_alloc(c, &P.kin, 6 * (size_t)cap);
   rc |= dev_alloc(c, &P.zone, 6 * (size_t)cap);
@@ -275,12 +274,10 @@ int ensure_particles(lbmpsm_ctx* c, int n) {
   rc |= dev_alloc(c, &P.out, 12 * (size_t)cap);
   if (rc) return LBMPSM_ERR_NOMEM;
   P.cap = cap;
-  // pinned staging: tag + 10 doubles per entry
-  const size_t bytes = (size_t)cap * (sizeof(int) + 10 * sizeof(double));
   if (c->h_stage) (void)hipHostFree(c->h_stage);
   c->h_stage = nullptr;
-  HIPCHK(c, hipHostMalloc(&c->h_stage, bytes));
-  c->h_stage_bytes = bytes;
+  HIPCHK(c, hipHostMalloc(&c->h_stage, 84 * (size_t)cap));
+  c->h_stage_bytes = 84 * (size_t)cap;
   return LBMPSM_OK;
 }
 
@@ -348,13 +345,14 @@ void build_faces(lbmpsm_ctx* c, int typeBC, double U, double rho_outlet) {
 int run_map(lbmpsm_ctx* c) {
   ParticleDev& P = c->part;
   BrickDev& B = c->br;
+  const int n = P.n;
   PrepArgs pa{};
   pa.g = c->g;
-  pa.n = P.n;
-  pa.x = P.x;
-  pa.v = P.v;
-  pa.w = P.w;
-  pa.r = P.r;
+  pa.n = n;
+  pa.x = P.x(n);
+  pa.v = P.v(n);
+  pa.w = P.w(n);
+  pa.r = P.r(n);
   pa.xl = P.xl;
   pa.kin = P.kin;
   pa.zone = P.zone;
@@ -366,6 +364,7 @@ int run_map(lbmpsm_ctx* c) {
   pa.tc = c->cfg.tc;
   pa.dt_d = c->cfg.dt_d;
   pa.count = B.count;
+  pa.list = B.list;
   pa.bi = B.bi;
   pa.bj = B.bj;
   pa.bk = B.bk;
@@ -375,22 +374,18 @@ int run_map(lbmpsm_ctx* c) {
   {
     ProfScope ps(c, "map_bin");
     HIPCHK(c, launch_fill_int(B.count, B.nbricks, 0, c->stream));
-    HIPCHK(c, launch_prep_count(pa, c->stream));
-    HIPCHK(c, launch_scan_bricks(B, c->stream));
-    HIPCHK(c, launch_fill_lists(pa, B, c->stream));
+    HIPCHK(c, launch_prep_bin(pa, c->stream));
   }
   MapArgs ma{};
   ma.g = c->g;
   ma.s = c->slots;
-  ma.tag = P.tag;
+  ma.n = n;
+  ma.tag = P.tag(n);
   ma.xl = P.xl;
   ma.kin = P.kin;
   ma.zone = P.zone;
   ma.list = B.list;
-  ma.offset = B.offset;
   ma.count = B.count;
-  ma.nonempty = B.nonempty;
-  ma.nonempty_n = B.nonempty_n;
   ma.bi = B.bi;
   ma.bj = B.bj;
   ma.bk = B.bk;
@@ -398,21 +393,16 @@ int run_map(lbmpsm_ctx* c) {
   ma.nbj = B.nbj;
   ma.nbk = B.nbk;
   ma.hsq_half = c->g.dim == 3 ? std::sqrt(3.1) / 2.0 : std::sqrt(2.1) / 2.0;
-  const int grid = (int)std::min<int64_t>(B.nbricks, 4096);
+  ma.sfcode = c->sfcode;
+  ma.zone_cap = c->zone_cap;
+  {
+    ProfScope ps(c, "map_fraction");
+    HIPCHK(c, launch_entry_sf(ma, c->stream));
+  }
   {
     ProfScope ps(c, "map_particles");
-    HIPCHK(c, launch_map(ma, B.overflow, c->stream, grid));
+    HIPCHK(c, launch_map(ma, B.nbricks, c->stream));
   }
-  return LBMPSM_OK;
-}
-
-int check_overflow(lbmpsm_ctx* c) {
-  int ov = 0;
-  HIPCHK(c, hipMemcpyAsync(&ov, c->br.overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (ov)
-    return fail(c, LBMPSM_ERR_LIMIT,
-                "more than 2048 particle zones overlap one 256-node brick; the map was not applied");
   return LBMPSM_OK;
 }
 
@@ -577,13 +567,8 @@ int lbmpsm_create(const lbmpsm_config* cfg, lbmpsm_ctx** out) {
   B.nbj = (g.sy + B.bj - 1) / B.bj;
   B.nbk = (g.sz + B.bk - 1) / B.bk;
   B.nbricks = (int64_t)B.nbi * B.nbj * B.nbk;
-  rc |= dev_alloc(c, &B.count, B.nbricks + 1);
-  rc |= dev_alloc(c, &B.offset, B.nbricks + 1);
-  rc |= dev_alloc(c, &B.cursor, B.nbricks + 1);
-  rc |= dev_alloc(c, &B.nonempty, B.nbricks + 1);
-  rc |= dev_alloc(c, &B.nonempty_n, 1);
-  rc |= dev_alloc(c, &B.block_sums, (B.nbricks + 1023) / 1024 + 1);
-  rc |= dev_alloc(c, &B.overflow, 1);
+  rc |= dev_alloc(c, &B.count, B.nbricks);
+  rc |= dev_alloc(c, &B.list, (size_t)B.nbricks * kBrickCap);
   if (rc) {
     std::string m = c->err;
     lbmpsm_destroy(c);
@@ -592,10 +577,6 @@ int lbmpsm_create(const lbmpsm_config* cfg, lbmpsm_ctx** out) {
   if (hipEventCreateWithFlags(&c->stage_done, hipEventDisableTiming) != hipSuccess) {
     lbmpsm_destroy(c);
     return fail(nullptr, LBMPSM_ERR_HIP, "hipEventCreate failed");
-  }
-  if (launch_fill_int(B.overflow, 1, 0, c->stream) != hipSuccess) {
-    lbmpsm_destroy(c);
-    return fail(nullptr, LBMPSM_ERR_HIP, "initial fill failed");
   }
   *out = c;
   // reference start: rho = 1, u = 0 (src/fix_LBM_PSM.cpp:168)
@@ -623,12 +604,13 @@ int lbmpsm_destroy(lbmpsm_ctx* c) {
   dev_free(c->rho);
   dev_free(c->u);
   ParticleDev& P = c->part;
-  void* pp[] = {P.tag, P.x, P.v, P.w, P.r, P.xl, P.kin, P.zone, P.rzone, P.out};
+  void* pp[] = {P.raw, P.xl, P.kin, P.zone, P.rzone, P.out};
   for (void* p : pp) dev_free(p);
   BrickDev& B = c->br;
-  void* bb[] = {B.count, B.offset, B.cursor, B.nonempty, B.nonempty_n, B.block_sums, B.list, B.overflow};
+  void* bb[] = {B.count, B.list};
   for (void* p : bb) dev_free(p);
   dev_free(c->d_scratch);
+  dev_free(c->sfcode);
   dev_free(c->d_cb);
   dev_free(c->d_faces);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -705,47 +687,45 @@ int lbmpsm_set_particles(lbmpsm_ctx* c, int n, const int* tag, const double* x, 
   if (n == 0) return LBMPSM_OK;
   int rc = ensure_particles(c, n);
   if (rc) return rc;
-  // bound on (entry, brick) pairs for the list buffer
-  BrickDev& B = c->br;
-  int64_t pairs = 0;
-  const int bmin = std::min(B.bi, std::min(B.bj, B.bk));
+  // codes per entry: the largest zone box, (2 r/dx + 8) nodes per axis bounds
+  // every map zone (src/LBM_PSM_exchangeParticleData.cpp:41-48)
+  double rmax = 0.0;
   for (int p = 0; p < n; ++p) {
-    const double span = 2.0 * std::fabs(radius[p]) / c->cfg.dx + 8.0;
-    if (!(span < 1e9)) return fail(c, LBMPSM_ERR_ARG, "radius is not finite");
-    int64_t per[3];
-    const int be[3] = {B.bi, B.bj, B.bk};
-    const int nb[3] = {B.nbi, B.nbj, B.nbk};
-    for (int d = 0; d < 3; ++d) per[d] = std::min<int64_t>((int64_t)(span / be[d]) + 2, nb[d]);
-    pairs += per[0] * per[1] * per[2];
-    (void)bmin;
+    if (!(std::fabs(radius[p]) < 1e300)) return fail(c, LBMPSM_ERR_ARG, "radius is not finite");
+    rmax = std::max(rmax, std::fabs(radius[p]));
   }
-  if (pairs > B.list_cap) {
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    const int64_t cap = std::max<int64_t>(pairs, 2 * B.list_cap);
-    dev_free(B.list);
-    c->device_bytes -= B.list_cap * (int64_t)sizeof(int);
-    B.list = nullptr;
-    B.list_cap = 0;
-    rc = dev_alloc(c, &B.list, (size_t)cap);
-    if (rc) return rc;
-    B.list_cap = cap;
+  {
+    const double span = 2.0 * rmax / c->cfg.dx + 8.0;
+    int64_t ext[3];
+    const int se[3] = {c->g.sx, c->g.sy, c->g.sz};
+    for (int d = 0; d < 3; ++d) ext[d] = std::min<int64_t>((int64_t)span, se[d]);
+    int64_t cap = ext[0] * ext[1] * ext[2];
+    // zones larger than 2^21 nodes are computed inside the map kernel instead
+    cap = std::min<int64_t>(cap, (int64_t)1 << 21);
+    const size_t need = (size_t)cap * (size_t)n;
+    if (need > c->sfcode_elems) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      dev_free(c->sfcode);
+      c->device_bytes -= (int64_t)(c->sfcode_elems * sizeof(unsigned short));
+      c->sfcode = nullptr;
+      c->sfcode_elems = 0;
+      const size_t alloc = std::max(need, c->sfcode_elems * 2);
+      rc = dev_alloc(c, &c->sfcode, alloc);
+      if (rc) return rc;
+      c->sfcode_elems = alloc;
+    }
+    c->zone_cap = (int)cap;
   }
-  // stage into pinned memory, then one async copy per array
+  // stage into pinned memory (after the previous upload has left it), one copy
   HIPCHK(c, hipEventSynchronize(c->stage_done));
   char* h = (char*)c->h_stage;
-  int* ht = (int*)h;
-  double* hd = (double*)(h + (size_t)c->part.cap * sizeof(int));
-  std::memcpy(ht, tag, n * sizeof(int));
-  std::memcpy(hd, x, 3 * n * sizeof(double));
-  std::memcpy(hd + 3 * (size_t)n, v, 3 * n * sizeof(double));
-  std::memcpy(hd + 6 * (size_t)n, omega, 3 * n * sizeof(double));
-  std::memcpy(hd + 9 * (size_t)n, radius, n * sizeof(double));
-  ParticleDev& P = c->part;
-  HIPCHK(c, hipMemcpyAsync(P.tag, ht, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(P.x, hd, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(P.v, hd + 3 * (size_t)n, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(P.w, hd + 6 * (size_t)n, 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(P.r, hd + 9 * (size_t)n, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  double* hd = (double*)h;
+  std::memcpy(hd, x, 3 * (size_t)n * sizeof(double));
+  std::memcpy(hd + 3 * (size_t)n, v, 3 * (size_t)n * sizeof(double));
+  std::memcpy(hd + 6 * (size_t)n, omega, 3 * (size_t)n * sizeof(double));
+  std::memcpy(hd + 9 * (size_t)n, radius, (size_t)n * sizeof(double));
+  std::memcpy(h + 80 * (size_t)n, tag, (size_t)n * sizeof(int));
+  HIPCHK(c, hipMemcpyAsync(c->part.raw, h, 84 * (size_t)n, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipEventRecord(c->stage_done, c->stream));
   return run_map(c);
 }
@@ -764,13 +744,17 @@ int lbmpsm_step(lbmpsm_ctx* c, int flags) {
     ProfScope ps(c, "collide_stream");
     HIPCHK(c, launch_collide(c->cfg.storage, a, c->stream));
   }
+  if (a.nfaces && !a.poststream) {
+    ProfScope ps(c, "collide_faces");
+    HIPCHK(c, launch_collide_faces(c->cfg.storage, a, c->stream));
+  }
   c->cur = 1 - c->cur;
   c->poststream = 0;
   if (store) c->have_macros = 1;
   HydroArgs h{};
   h.g = c->g;
   h.n = c->part.n;
-  h.tag = c->part.tag;
+  h.tag = c->part.tag(c->part.n);
   h.xl = c->part.xl;
   h.rzone = c->part.rzone;
   h.slot_tag = c->slots.tag;
@@ -792,10 +776,8 @@ int lbmpsm_get_hydro(lbmpsm_ctx* c, int n, double* F, double* T, double* S) {
   HIPCHK(c, hipSetDevice(c->device));
   if (!c->hydro_ready) return fail(c, LBMPSM_ERR_STATE, "no step has run since the particles were set");
   if (n != c->hydro_n) return fail(c, LBMPSM_ERR_ARG, "n differs from the particle count of the last step");
-  int rc = check_overflow(c);
-  if (rc) return rc;
   if (n == 0) return LBMPSM_OK;
-  rc = ensure_host_out(c, (size_t)n * 12 * sizeof(double));
+  int rc = ensure_host_out(c, (size_t)n * 12 * sizeof(double));
   if (rc) return rc;
   HIPCHK(c, hipMemcpyAsync(c->h_out, c->part.out, (size_t)n * 12 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -17,6 +17,22 @@
 
 #include "lbmpsm_internal.h"
 
+// Tuning knobs of the streaming kernels (compile-time).
+#ifndef LBMPSM_NT_STORE
+#define LBMPSM_NT_STORE 1  // non-temporal population stores
+#endif
+#ifndef LBMPSM_NT_LOAD
+#define LBMPSM_NT_LOAD 0   // non-temporal population loads
+#endif
+#ifndef LBMPSM_WAVES_PER_EU
+#define LBMPSM_WAVES_PER_EU 0  // > 0: ask for this many waves per SIMD
+#endif
+#if LBMPSM_WAVES_PER_EU > 0
+#define LBMPSM_OCC __attribute__((amdgpu_waves_per_eu(LBMPSM_WAVES_PER_EU)))
+#else
+#define LBMPSM_OCC
+#endif
+
 namespace lbmpsm {
 
 enum { ST_F64 = 0, ST_F32 = 1, ST_F16 = 2 };
@@ -46,7 +62,11 @@ __device__ __forceinline__ typename StoreT<ST>::C ld(const typename StoreT<ST>::
     return __half2float(*p) * inv_scale;
   } else {
     (void)inv_scale;
+#if LBMPSM_NT_LOAD
+    return __builtin_nontemporal_load(p);
+#else
     return *p;
+#endif
   }
 }
 
@@ -57,7 +77,11 @@ __device__ __forceinline__ void st(typename StoreT<ST>::T* p, typename StoreT<ST
     *p = __float2half_rn(v * scale);
   } else {
     (void)scale;
+#if LBMPSM_NT_STORE
     __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
   }
 }
 
@@ -102,22 +126,24 @@ struct Row {
 };
 
 template <int DIM>
-__device__ __forceinline__ bool make_row(const Geom& g, int64_t wave, Row& r) {
+__device__ __forceinline__ bool make_row(const Geom& g, int64_t wave64, Row& r) {
+  // 32-bit arithmetic: the stored box has fewer than 2^31 nodes
   bool vim, vip, vjm = true, vjp = true;
+  const int wave = (int)wave64;
   const int ncols = DIM == 3 ? g.sz : g.sy;
   const int nch = (ncols + 63) >> 6;
-  const int64_t row = wave / nch;
-  r.c0 = (int)(wave - row * nch) * 64;
+  const int row = wave / nch;
+  r.c0 = (wave - row * nch) * 64;
   r.ncols = ncols;
   if constexpr (DIM == 3) {
-    if (row >= (int64_t)g.sx * g.sy) return false;
-    r.i = (int)(row / g.sy);
-    r.j = (int)(row - (int64_t)r.i * g.sy);
+    if (row >= g.sx * g.sy) return false;
+    r.i = row / g.sy;
+    r.j = row - r.i * g.sy;
     nbr(r.i, g.sx, g.per[0], r.im, vim, r.ip, vip);
     nbr(r.j, g.sy, g.per[1], r.jm, vjm, r.jp, vjp);
   } else {
     if (row >= g.sx) return false;
-    r.i = (int)row;
+    r.i = row;
     r.j = 0;
     r.jm = r.jp = 0;
     nbr(r.i, g.sx, g.per[0], r.im, vim, r.ip, vip);
@@ -228,7 +254,7 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
 
 // Bulk: every node that is on no Zou-He face.
 template <int DIM, int Q>
-__global__ __launch_bounds__(64) void k_collide_f64(const CollideArgs a) {
+__global__ __launch_bounds__(64) LBMPSM_OCC void k_collide_f64(const CollideArgs a) {
   const Geom& g = a.g;
   Row r;
   if (!make_row<DIM>(g, blockIdx.x, r)) return;
@@ -266,7 +292,11 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
     const double usqt = 0.5 * (ux * ux + uy * uy + uz * uz) * inv2;
     const uint8_t m = a.mask[n];
 
-    if (m == 0) {
+    // A wave with any particle node runs the particle form for all its
+    // lanes: with both slots empty it yields the fluid value (the extra
+    // terms add +0.0).  The branch is wave-uniform, so every population
+    // store below is issued once by all 64 lanes together.
+    if (!__any(m != 0)) {
       Unroll<0, Q>::run([&]<int q>() {
         constexpr double w = VelSet<Q>::w[q];
         const double eu = edot<Q, q>(ux, uy, uz);
@@ -382,7 +412,7 @@ __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, uns
                                              unsigned uc);
 
 template <int DIM, int Q, int ST>
-__global__ __launch_bounds__(64) void k_collide_lowp(const CollideArgs a) {
+__global__ __launch_bounds__(64) LBMPSM_OCC void k_collide_lowp(const CollideArgs a) {
   const Geom& g = a.g;
   Row r;
   if (!make_row<DIM>(g, blockIdx.x, r)) return;
@@ -423,8 +453,9 @@ __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, uns
     }
     const float usq = 1.5f * (ux * ux + uy * uy + uz * uz);
     const uint8_t m = a.mask[n];
-    // g_eq = w (drho + rho (3 e.u + 4.5 (e.u)^2 - 1.5 u^2))
-    if (m == 0) {
+    // g_eq = w (drho + rho (3 e.u + 4.5 (e.u)^2 - 1.5 u^2)); mixed waves run
+    // the particle form for all lanes (wave-uniform branch, whole-wave stores)
+    if (!__any(m != 0)) {
       Unroll<0, Q>::run([&]<int q>() {
         constexpr float w = (float)VelSet<Q>::w[q];
         const float eu = edot<Q, q>(ux, uy, uz);
@@ -650,7 +681,12 @@ static hipError_t collide_dim(int storage, const CollideArgs& a, hipStream_t s) 
     case ST_F16: hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16>), grid, dim3(64), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
-  if (a.poststream) return hipGetLastError();
+  return hipGetLastError();
+}
+
+template <int DIM, int Q>
+static hipError_t collide_faces_dim(int storage, const CollideArgs& a, hipStream_t s) {
+  if (a.poststream) return hipSuccess;
   for (int fi = 0; fi < a.nfaces; ++fi) {
     const Face& F = a.host_faces[fi];
     const int64_t cnt = (int64_t)(F.a_hi - F.a_lo + 1) * (F.b_hi - F.b_lo + 1);
@@ -668,6 +704,11 @@ static hipError_t collide_dim(int storage, const CollideArgs& a, hipStream_t s) 
 
 hipError_t launch_collide(int storage, const CollideArgs& a, hipStream_t s) {
   return a.g.dim == 3 ? collide_dim<3, 19>(storage, a, s) : collide_dim<2, 9>(storage, a, s);
+}
+
+hipError_t launch_collide_faces(int storage, const CollideArgs& a, hipStream_t s) {
+  return a.g.dim == 3 ? collide_faces_dim<3, 19>(storage, a, s)
+                      : collide_faces_dim<2, 9>(storage, a, s);
 }
 
 template <int DIM, int Q>

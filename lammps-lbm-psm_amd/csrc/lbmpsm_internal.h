@@ -54,37 +54,36 @@ struct CollideArgs {
   const Face* host_faces;  // the same table in host memory (launch sizes only)
 };
 
-// Particle list state on the device (SoA, capacity entries).
+// Particle list state on the device.  The raw LAMMPS arrays of one call
+// arrive in one copy: x[n][3], v[n][3], omega[n][3], radius[n] (doubles)
+// followed by tag[n] (ints), i.e. 84 bytes per entry.
 struct ParticleDev {
   int cap = 0;
   int n = 0;
-  // host->device staging (raw LAMMPS data)
-  int* tag = nullptr;
-  double* x = nullptr;      // [n][3]
-  double* v = nullptr;      // [n][3]
-  double* w = nullptr;      // [n][3]
-  double* r = nullptr;      // [n]
+  char* raw = nullptr;      // [cap * 84] bytes
   // derived
   double* xl = nullptr;     // [n][4]  x_loc, y_loc, z_loc (reference halo'd coords), r_lb
   double* kin = nullptr;    // [n][6]  uTrans, omegaLB
   int* zone = nullptr;      // [n][6]  stored-coordinate map zone [lo,hi) x,y,z
   int* rzone = nullptr;     // [n][6]  stored-coordinate reduction zone
   double* out = nullptr;    // [n][12] F, T, S (physical units)
+  const double* x(int nn) const { return (const double*)raw; }
+  const double* v(int nn) const { return (const double*)raw + 3 * (size_t)nn; }
+  const double* w(int nn) const { return (const double*)raw + 6 * (size_t)nn; }
+  const double* r(int nn) const { return (const double*)raw + 9 * (size_t)nn; }
+  const int* tag(int nn) const { return (const int*)(raw + 80 * (size_t)nn); }
 };
 
+// Node bricks of the particle map: per brick, the list entries whose map
+// zone touches it (unordered, at most kBrickCap; longer lists make the map
+// kernel scan the whole particle list for that brick).
+constexpr int kBrickCap = 64;
 struct BrickDev {
   int bi = 0, bj = 0, bk = 0;  // brick edge lengths (nodes)
   int nbi = 0, nbj = 0, nbk = 0;
   int64_t nbricks = 0;
-  int* count = nullptr;     // [nbricks + 1]
-  int* offset = nullptr;    // [nbricks + 1]
-  int* cursor = nullptr;    // [nbricks]
-  int* nonempty = nullptr;  // [nbricks]
-  int* nonempty_n = nullptr;  // [1]
-  int* block_sums = nullptr;  // scan scratch
-  int* list = nullptr;      // [list_cap]
-  int64_t list_cap = 0;
-  int* overflow = nullptr;  // [1] lists longer than the map kernel's LDS
+  int* count = nullptr;     // [nbricks]
+  int* list = nullptr;      // [nbricks][kBrickCap]
 };
 
 struct SlotsDev {
@@ -95,26 +94,32 @@ struct SlotsDev {
   uint8_t* mask = nullptr;  // [plane]
 };
 
+// Solid-fraction code of an entry at a node of its map zone: the
+// sub-point count (fraction * count), or kSfCodeOne for the reference's
+// early "fully inside" exit (exactly 1.0).
+constexpr unsigned short kSfCodeOne = 0xFFFF;
+
 struct MapArgs {
   Geom g;
   SlotsDev s;
+  int n;
   const int* tag;
   const double* xl;
   const double* kin;
   const int* zone;
   const int* list;
-  const int* offset;
   const int* count;
-  const int* nonempty;
-  const int* nonempty_n;
+  unsigned short* sfcode;  // [n][zone_cap] codes of each entry's zone, k fastest
+  int zone_cap;            // codes per entry
   int bi, bj, bk, nbi, nbj, nbk;
   double hsq_half;
 };
 
-constexpr int kMapMaxList = 2048;  // particles per brick the map kernel sorts in LDS
-
 // ---- launchers (lattice_f64.hip / lattice_lowp.hip) ------------------------
+// Bulk nodes (every node on no Zou-He face) ...
 hipError_t launch_collide(int storage, const CollideArgs& a, hipStream_t s);
+// ... and the face nodes, with the faces applied after the pull.
+hipError_t launch_collide_faces(int storage, const CollideArgs& a, hipStream_t s);
 hipError_t launch_fill_planes(int storage, void* f, const Geom& g, const double* cb,
                               double rho0, float scale, hipStream_t s);
 // Post-stream + BC values of interior nodes into `out` (FP64, reference
@@ -142,13 +147,16 @@ struct PrepArgs {
   double origin[3];
   double dx, Uc, u_lb, tc, dt_d;
   int* count;
+  int* list;
   int bi, bj, bk, nbi, nbj, nbk;
 };
 hipError_t launch_fill_int(int* p, int64_t n, int value, hipStream_t s);
-hipError_t launch_prep_count(const PrepArgs& a, hipStream_t s);
-hipError_t launch_scan_bricks(BrickDev& b, hipStream_t s);
-hipError_t launch_fill_lists(const PrepArgs& a, const BrickDev& b, hipStream_t s);
-hipError_t launch_map(const MapArgs& a, int* overflow, hipStream_t s, int grid);
+// Per-entry lattice quantities and brick lists (one wavefront per entry).
+hipError_t launch_prep_bin(const PrepArgs& a, hipStream_t s);
+// Solid fraction of every entry at every node of its map zone.
+hipError_t launch_entry_sf(const MapArgs& a, hipStream_t s);
+// Slot state machine on every touched brick.
+hipError_t launch_map(const MapArgs& a, int64_t nbricks, hipStream_t s);
 struct HydroArgs {
   Geom g;
   int n;

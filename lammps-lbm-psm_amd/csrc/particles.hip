@@ -1,9 +1,9 @@
 // Particle <-> lattice coupling kernels.
 //
-//  * prep: per list entry, the reference's lattice coordinates, radius,
-//    node-velocity terms and zones (src/LBM_PSM_exchangeParticleData.cpp:31-55
-//    and :175-195), plus a count of the node bricks each map zone touches.
-//  * scan / fill: per-brick lists of the entries whose zone touches it.
+//  * prep/bin: one wavefront per list entry computes the reference's lattice
+//    coordinates, radius, node-velocity terms and zones
+//    (src/LBM_PSM_exchangeParticleData.cpp:31-55 and :175-195) and appends
+//    the entry to the list of every node brick its map zone touches.
 //  * map: one workgroup per touched brick.  The reference visits particles in
 //    list order and, for each, the nodes of its zone; a node's slots depend
 //    only on the ordered sequence of entries whose zone contains it.  Each
@@ -55,29 +55,18 @@ __device__ __forceinline__ bool zone_empty(const int* z) {
   return z[1] <= z[0] || z[3] <= z[2] || z[5] <= z[4];
 }
 
-__global__ __launch_bounds__(256) void k_prep_count(const PrepArgs a) {
-  const int p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= a.n) return;
+// Reference lattice quantities of entry p; zones in stored coordinates.
+__device__ __forceinline__ void entry_geometry(const PrepArgs& a, int p, double loc[3],
+                                               double& r_lb, int z[6], int rz[6]) {
   const Geom& g = a.g;
   // src/LBM_PSM_exchangeParticleData.cpp:31-39 on one rank: the local shift
   // (nTotal - coord*2*envelope - 1) evaluates to -1.0 on every axis.
   const double off = (0 - ((double)0 * 2.0 * (double)1) - 1);
-  double loc[3];
   for (int d = 0; d < 3; ++d) loc[d] = (a.x[3 * p + d] - a.origin[d]) / a.dx - off;
-  const double r_lb = a.r[p] / a.dx;
-  a.xl[4 * p + 0] = loc[0];
-  a.xl[4 * p + 1] = loc[1];
-  a.xl[4 * p + 2] = loc[2];
-  a.xl[4 * p + 3] = r_lb;
+  r_lb = a.r[p] / a.dx;
   for (int d = 0; d < 3; ++d) {
-    a.kin[6 * p + d] = a.v[3 * p + d] / a.Uc * a.u_lb;      // get_vel_lb
-    a.kin[6 * p + 3 + d] = a.w[3 * p + d] * a.tc * a.dt_d;  // get_freq_lb
-  }
-  int* z = a.zone + 6 * p;
-  int* rz = a.rzone + 6 * p;
-  for (int d = 0; d < 3; ++d) {
-    int lo = (int)(loc[d] - r_lb) - 2;
-    int hi = (int)(loc[d] + r_lb) + 2;
+    const int lo = (int)(loc[d] - r_lb) - 2;
+    const int hi = (int)(loc[d] + r_lb) + 2;
     const int nh = ref_extent(g, d);
     // map zone, :44-48
     int mlo, mhi;
@@ -103,204 +92,301 @@ __global__ __launch_bounds__(256) void k_prep_count(const PrepArgs a) {
     rz[2 * d] = clampi(rlo - g.off[d], 0, se);
     rz[2 * d + 1] = clampi(rhi - g.off[d], 0, se);
   }
-  if (zone_empty(z)) return;
-  int b[6];
-  brick_range(z, a.bi, a.bj, a.bk, b);
-  for (int x = b[0]; x <= b[1]; ++x)
-    for (int y = b[2]; y <= b[3]; ++y)
-      for (int w = b[4]; w <= b[5]; ++w)
-        atomicAdd(&a.count[((int64_t)x * a.nbj + y) * a.nbk + w], 1);
 }
 
-// Exclusive scan of brick counts: 1024 per block, then block sums.
-__global__ __launch_bounds__(1024) void k_scan_block(const int* count, int* offset, int* sums,
-                                                     int64_t n) {
-  __shared__ int s[1024];
-  const int64_t t = (int64_t)blockIdx.x * 1024 + threadIdx.x;
-  const int v = t < n ? count[t] : 0;
-  s[threadIdx.x] = v;
-  __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    int add = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
-    __syncthreads();
-    s[threadIdx.x] += add;
-    __syncthreads();
-  }
-  if (t < n) offset[t] = s[threadIdx.x] - v;
-  if (threadIdx.x == 1023) sums[blockIdx.x] = s[1023];
-}
-
-__global__ __launch_bounds__(1024) void k_scan_sums(int* sums, int nb) {
-  __shared__ int s[1024];
-  // sequential carry over chunks of 1024 block sums
-  int carry = 0;
-  for (int base = 0; base < nb; base += 1024) {
-    const int t = base + threadIdx.x;
-    const int v = t < nb ? sums[t] : 0;
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-      int add = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
-      __syncthreads();
-      s[threadIdx.x] += add;
-      __syncthreads();
-    }
-    if (t < nb) sums[t] = carry + s[threadIdx.x] - v;
-    const int tot = s[1023];
-    __syncthreads();
-    carry += tot;
-  }
-}
-
-__global__ __launch_bounds__(1024) void k_scan_add(const int* count, int* offset,
-                                                   const int* sums, int* cursor,
-                                                   int* nonempty, int* nonempty_n, int64_t n) {
-  const int64_t t = (int64_t)blockIdx.x * 1024 + threadIdx.x;
-  if (t >= n) return;
-  offset[t] += sums[blockIdx.x];
-  cursor[t] = 0;
-  if (count[t] > 0) {
-    const int slot = atomicAdd(nonempty_n, 1);
-    nonempty[slot] = (int)t;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_fill_lists(const PrepArgs a, const int* offset,
-                                                    int* cursor, int* list) {
-  const int p = blockIdx.x * 256 + threadIdx.x;
+// One wavefront per entry: every lane derives the same geometry, lane 0
+// stores it, and the lanes share the appends to the touched bricks.
+__global__ __launch_bounds__(64) void k_prep_bin(const PrepArgs a) {
+  const int p = blockIdx.x;
   if (p >= a.n) return;
-  const int* z = a.zone + 6 * p;
+  double loc[3], r_lb;
+  int z[6], rz[6];
+  entry_geometry(a, p, loc, r_lb, z, rz);
+  if (threadIdx.x == 0) {
+    a.xl[4 * p + 0] = loc[0];
+    a.xl[4 * p + 1] = loc[1];
+    a.xl[4 * p + 2] = loc[2];
+    a.xl[4 * p + 3] = r_lb;
+    for (int d = 0; d < 3; ++d) {
+      a.kin[6 * p + d] = a.v[3 * p + d] / a.Uc * a.u_lb;      // get_vel_lb
+      a.kin[6 * p + 3 + d] = a.w[3 * p + d] * a.tc * a.dt_d;  // get_freq_lb
+    }
+    for (int c = 0; c < 6; ++c) {
+      a.zone[6 * p + c] = z[c];
+      a.rzone[6 * p + c] = rz[c];
+    }
+  }
   if (zone_empty(z)) return;
   int b[6];
   brick_range(z, a.bi, a.bj, a.bk, b);
-  for (int x = b[0]; x <= b[1]; ++x)
-    for (int y = b[2]; y <= b[3]; ++y)
-      for (int w = b[4]; w <= b[5]; ++w) {
-        const int64_t id = ((int64_t)x * a.nbj + y) * a.nbk + w;
-        const int pos = atomicAdd(&cursor[id], 1);
-        list[offset[id] + pos] = p;
-      }
+  const int nx = b[1] - b[0] + 1, ny = b[3] - b[2] + 1, nz = b[5] - b[4] + 1;
+  const int tot = nx * ny * nz;
+  for (int t = threadIdx.x; t < tot; t += 64) {
+    const int x = b[0] + t / (ny * nz);
+    const int y = b[2] + (t / nz) % ny;
+    const int w = b[4] + t % nz;
+    const int64_t id = ((int64_t)x * a.nbj + y) * a.nbk + w;
+    const int slot = atomicAdd(&a.count[id], 1);
+    if (slot < kBrickCap) a.list[id * kBrickCap + slot] = p;
+  }
 }
 
-// One workgroup (256 threads = one brick of 256 nodes) per touched brick.
-__global__ __launch_bounds__(256) void k_map(const MapArgs a, int* overflow) {
-  __shared__ int raw[kMapMaxList];
-  __shared__ int sorted[kMapMaxList];
+// Per-entry data the map kernel stages in LDS.
+struct MapEntry {
+  int zone[6];
+  int tag;
+  double xl[4];
+  double kin[6];
+};
+
+// One state-machine transition of a node for one entry whose zone contains
+// it, given the entry's solid fraction at the node,
+// src/LBM_PSM_exchangeParticleData.cpp:69-106.  Solid fractions of the
+// node's slots are read from memory only when the entry's tag is found in
+// a slot (the only case that uses them).
+struct NodeSlots {
+  int t0, t1;
+  double sf0, sf1;
+  double u0[3], u1[3];
+  bool d0, d1;      // slot written this step
+  bool k0, k1;      // sf0 / sf1 known (loaded or written)
+};
+
+__device__ __forceinline__ void map_transition(const MapArgs& a, const MapEntry& E, double sf,
+                                               int ri, int rj, int rk, int n, NodeSlots& S) {
+  const int64_t plane = a.g.plane;
+  const int tag = E.tag;
+  int id_old = 0;
+  double sf_old = 0.0;
+  if (tag == S.t0) {
+    if (!S.k0) {
+      S.sf0 = a.s.sf[n];
+      S.k0 = true;
+    }
+    id_old = S.t0;
+    sf_old = S.sf0;
+  } else if (tag == S.t1) {
+    if (!S.k1) {
+      S.sf1 = a.s.sf[plane + n];
+      S.k1 = true;
+    }
+    id_old = S.t1;
+    sf_old = S.sf1;
+  }
+  const int dec = (sf > 0.00001) + 2 * (sf_old > 0.00001);
+  bool do_set = false, do_clear = false;
+  switch (dec) {
+    case 0: do_clear = true; break;
+    case 1: do_set = true; break;
+    case 2: do_clear = (id_old == tag); break;
+    case 3: do_set = (sf > sf_old || id_old == tag); break;
+  }
+  if (do_set) {  // setParticleOnLattice, src/LBM_PSM_lattice.cpp:257-299
+    const double dx = (double)ri - E.xl[0];
+    const double dy = (double)rj - E.xl[1];
+    const double dz = (double)rk - E.xl[2];
+    const double* kn = E.kin;
+    double un[3];
+    un[0] = kn[0] + kn[4] * dz - kn[5] * dy;
+    un[1] = kn[1] + kn[5] * dx - kn[3] * dz;
+    un[2] = kn[2] + kn[3] * dy - kn[4] * dx;
+    int sl = -1;
+    if (S.t0 == tag) sl = 0;
+    else if (S.t1 == tag) sl = 1;
+    else if (S.t0 == 0) sl = 0;
+    else if (S.t1 == 0) sl = 1;
+    if (sl == 0) {
+      S.t0 = tag; S.sf0 = sf; S.k0 = true; S.d0 = true;
+      S.u0[0] = un[0]; S.u0[1] = un[1]; S.u0[2] = un[2];
+    } else if (sl == 1) {
+      S.t1 = tag; S.sf1 = sf; S.k1 = true; S.d1 = true;
+      S.u1[0] = un[0]; S.u1[1] = un[1]; S.u1[2] = un[2];
+    }
+  } else if (do_clear) {  // setToZero, src/LBM_PSM_lattice.cpp:302-324
+    if (S.t0 == tag) {
+      S.t0 = 0; S.sf0 = 0.0; S.k0 = true; S.d0 = true;
+      S.u0[0] = S.u0[1] = S.u0[2] = 0.0;
+    } else if (S.t1 == tag) {
+      S.t1 = 0; S.sf1 = 0.0; S.k1 = true; S.d1 = true;
+      S.u1[0] = S.u1[1] = S.u1[2] = 0.0;
+    }
+  }
+}
+
+// Solid fraction of one entry over its map zone.  Each workgroup takes one
+// chunk of kSfChunk consecutive zone nodes (grid = entries x chunks per
+// entry; chunks past the end of a zone exit):
+//  1. every thread classifies its nodes with the reference's early exits;
+//     cells that need the sub-point count go to a queue;
+//  2. groups of kMapGroup lanes count the queued cells, each lane taking a
+//     share of the 100 (a,b) sub-point pairs, and add the integer parts.
+// Codes are stored k fastest over the zone box (index within the zone).
+constexpr int kSfChunk = 1024;
+constexpr int kMapGroup = 8;
+
+__global__ __launch_bounds__(256) void k_entry_sf(const MapArgs a) {
+  __shared__ int qnode[kSfChunk];
+  __shared__ int qlen;
+  const int chunks = (a.zone_cap + kSfChunk - 1) / kSfChunk;
+  const int e = blockIdx.x / chunks;
+  const int base = (blockIdx.x - e * chunks) * kSfChunk;
+  if (e >= a.n) return;
+  const Geom& g = a.g;
+  const int* z = a.zone + 6 * e;
+  const int ni = z[1] - z[0], nj = z[3] - z[2], nk = z[5] - z[4];
+  if (ni <= 0 || nj <= 0 || nk <= 0) return;
+  const int tot = ni * nj * nk, njk = nj * nk;
+  if (tot > a.zone_cap || base >= tot) return;  // (oversized zones: see k_map)
+  const double xL = a.xl[4 * e + 0], yL = a.xl[4 * e + 1], zL = a.xl[4 * e + 2];
+  const double rL = a.xl[4 * e + 3];
+  unsigned short* out = a.sfcode + (size_t)e * a.zone_cap;
+  const int tid = threadIdx.x;
+  const int grp = tid / kMapGroup, gl = tid % kMapGroup;
+  {
+    if (tid == 0) qlen = 0;
+    __syncthreads();
+    const int cnt = min(kSfChunk, tot - base);
+    for (int t = tid; t < cnt; t += 256) {
+      const int li = (base + t) / njk;
+      const int rem = base + t - li * njk;
+      const int lj = rem / nk;
+      const int lk = rem - lj * nk;
+      const double dx = (double)(z[0] + li + g.off[0]) - xL;
+      const double dy = (double)(z[2] + lj + g.off[1]) - yL;
+      const double dz = (double)(z[4] + lk + g.off[2]) - zL;
+      const int cls = sf_classify(a.hsq_half, dx, dy, dz, rL);
+      if (cls == SF_ZERO) out[base + t] = 0;
+      else if (cls == SF_ONE) out[base + t] = kSfCodeOne;
+      else qnode[atomicAdd(&qlen, 1)] = base + t;
+    }
+    __syncthreads();
+    const int nq = qlen;
+    for (int q0 = 0; q0 < nq; q0 += 256 / kMapGroup) {
+      const int q = q0 + grp;
+      int part = 0, idx = 0;
+      if (q < nq) {
+        idx = qnode[q];
+        const int li = idx / njk;
+        const int rem = idx - li * njk;
+        const int lj = rem / nk;
+        const int lk = rem - lj * nk;
+        const double dx = (double)(z[0] + li + g.off[0]) - xL;
+        const double dy = (double)(z[2] + lj + g.off[1]) - yL;
+        const double dz = (double)(z[4] + lk + g.off[2]) - zL;
+        part = sf_count_pairs(g.dim, dx, dy, dz, rL, gl, kSfPairs, kMapGroup);
+      }
+      for (int off = kMapGroup / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, kMapGroup);
+      if (q < nq && gl == 0) out[idx] = (unsigned short)part;
+    }
+  }
+}
+
+// One workgroup (256 threads = one brick of 256 nodes) per brick.  The
+// entries to replay are taken in list order: the brick's own list, sorted,
+// or - when more entries touch the brick than its list holds - the whole
+// particle list, staged kMapChunk at a time.  Every node replays them with
+// the solid fractions from k_entry_sf.
+constexpr int kMapChunk = 32;
+
+__global__ __launch_bounds__(256) void k_map(const MapArgs a) {
+  __shared__ int order[kBrickCap];
+  __shared__ int raw[kBrickCap];
+  __shared__ MapEntry ent[kMapChunk];
+  __shared__ int entid[kMapChunk];
+  const int64_t b = blockIdx.x;
+  const int m = a.count[b];
+  if (m == 0) return;
   const Geom& g = a.g;
   const int64_t plane = g.plane;
-  const int nne = *a.nonempty_n;
-  for (int e = blockIdx.x; e < nne; e += gridDim.x) {
-    const int b = a.nonempty[e];
-    const int m = a.count[b];
-    const int base = a.offset[b];
-    if (m > kMapMaxList) {
-      if (threadIdx.x == 0) atomicOr(overflow, 1);
-      continue;  // uniform across the workgroup
-    }
-    for (int t = threadIdx.x; t < m; t += 256) raw[t] = a.list[base + t];
-    __syncthreads();
+  const bool listed = m <= kBrickCap;
+  const int tid = threadIdx.x;
+
+  const int bb = (int)b;
+  const int bx = bb / (a.nbj * a.nbk);
+  const int by = (bb / a.nbk) % a.nbj;
+  const int bz = bb % a.nbk;
+  const int lk = tid % a.bk;
+  const int lj = (tid / a.bk) % a.bj;
+  const int li = tid / (a.bk * a.bj);
+  const int i = bx * a.bi + li, j = by * a.bj + lj, k = bz * a.bk + lk;
+  const bool valid = i < g.sx && j < g.sy && k < g.sz;
+  const int n = valid ? (i * g.sy + j) * g.sz + k : 0;
+  NodeSlots S;
+  S.t0 = valid ? a.s.tag[n] : 0;
+  S.t1 = valid ? a.s.tag[plane + n] : 0;
+  S.sf0 = S.sf1 = 0.0;
+  S.d0 = S.d1 = S.k0 = S.k1 = false;
+  for (int c = 0; c < 3; ++c) S.u0[c] = S.u1[c] = 0.0;
+  // reference (halo'd) node indices, as used in the geometry
+  const int ri = i + g.off[0], rj = j + g.off[1], rk = k + g.off[2];
+
+  if (listed) {
     // entries are distinct, so rank = number of smaller entries
-    for (int t = threadIdx.x; t < m; t += 256) {
-      const int v = raw[t];
+    if (tid < m) raw[tid] = a.list[b * kBrickCap + tid];
+    __syncthreads();
+    if (tid < m) {
+      const int v = raw[tid];
       int r = 0;
       for (int u = 0; u < m; ++u) r += raw[u] < v;
-      sorted[r] = v;
+      order[r] = v;
     }
     __syncthreads();
-
-    const int bx = b / (a.nbj * a.nbk);
-    const int by = (b / a.nbk) % a.nbj;
-    const int bz = b % a.nbk;
-    const int lk = threadIdx.x % a.bk;
-    const int lj = (threadIdx.x / a.bk) % a.bj;
-    const int li = threadIdx.x / (a.bk * a.bj);
-    const int i = bx * a.bi + li, j = by * a.bj + lj, k = bz * a.bk + lk;
-    const bool valid = i < g.sx && j < g.sy && k < g.sz;
-    if (valid) {
-      const int n = (i * g.sy + j) * g.sz + k;
-      int t0 = a.s.tag[n], t1 = a.s.tag[plane + n];
-      double sf0 = a.s.sf[n], sf1 = a.s.sf[plane + n];
-      double u0[3] = {a.s.us[n], a.s.us[plane + n], a.s.us[2 * plane + n]};
-      double u1[3] = {a.s.us[3 * plane + n], a.s.us[4 * plane + n], a.s.us[5 * plane + n]};
-      bool d0 = false, d1 = false;
-      // reference (halo'd) node indices, as used in the geometry
-      const int ri = i + g.off[0], rj = j + g.off[1], rk = k + g.off[2];
-      for (int t = 0; t < m; ++t) {
-        const int p = sorted[t];
-        const int* z = a.zone + 6 * p;
-        if (i < z[0] || i >= z[1] || j < z[2] || j >= z[3] || k < z[4] || k >= z[5]) continue;
-        const double xL = a.xl[4 * p + 0], yL = a.xl[4 * p + 1], zL = a.xl[4 * p + 2];
-        const double rL = a.xl[4 * p + 3];
-        const int tag = a.tag[p];
-        const double sf = solid_fraction(g.dim, a.hsq_half, ri, rj, rk, xL, yL, zL, rL);
-        int id_old = 0;
-        double sf_old = 0.0;
-        if (tag == t0) {
-          id_old = t0;
-          sf_old = sf0;
-        } else if (tag == t1) {
-          id_old = t1;
-          sf_old = sf1;
-        }
-        const int dec = (sf > 0.00001) + 2 * (sf_old > 0.00001);
-        const double dx = (double)ri - xL;
-        const double dy = (double)rj - yL;
-        const double dz = (double)rk - zL;
-        const double* kn = a.kin + 6 * p;
-        double un[3];
-        un[0] = kn[0] + kn[4] * dz - kn[5] * dy;
-        un[1] = kn[1] + kn[5] * dx - kn[3] * dz;
-        un[2] = kn[2] + kn[3] * dy - kn[4] * dx;
-        bool do_set = false, do_clear = false;
-        switch (dec) {
-          case 0: do_clear = true; break;
-          case 1: do_set = true; break;
-          case 2: do_clear = (id_old == tag); break;
-          case 3: do_set = (sf > sf_old || id_old == tag); break;
-        }
-        if (do_set) {  // setParticleOnLattice, src/LBM_PSM_lattice.cpp:257-299
-          int s = -1;
-          if (t0 == tag) s = 0;
-          else if (t1 == tag) s = 1;
-          else if (t0 == 0) s = 0;
-          else if (t1 == 0) s = 1;
-          if (s == 0) {
-            t0 = tag; sf0 = sf; u0[0] = un[0]; u0[1] = un[1]; u0[2] = un[2]; d0 = true;
-          } else if (s == 1) {
-            t1 = tag; sf1 = sf; u1[0] = un[0]; u1[1] = un[1]; u1[2] = un[2]; d1 = true;
-          }
-        } else if (do_clear) {  // setToZero, src/LBM_PSM_lattice.cpp:302-324
-          if (t0 == tag) {
-            t0 = 0; sf0 = 0.0; u0[0] = u0[1] = u0[2] = 0.0; d0 = true;
-          } else if (t1 == tag) {
-            t1 = 0; sf1 = 0.0; u1[0] = u1[1] = u1[2] = 0.0; d1 = true;
-          }
-        }
-      }
-      if (d0) {
-        a.s.tag[n] = t0;
-        a.s.sf[n] = sf0;
-        a.s.us[n] = u0[0];
-        a.s.us[plane + n] = u0[1];
-        a.s.us[2 * plane + n] = u0[2];
-        a.s.Fh[n] = 0.0;
-        a.s.Fh[plane + n] = 0.0;
-        a.s.Fh[2 * plane + n] = 0.0;
-      }
-      if (d1) {
-        a.s.tag[plane + n] = t1;
-        a.s.sf[plane + n] = sf1;
-        a.s.us[3 * plane + n] = u1[0];
-        a.s.us[4 * plane + n] = u1[1];
-        a.s.us[5 * plane + n] = u1[2];
-        a.s.Fh[3 * plane + n] = 0.0;
-        a.s.Fh[4 * plane + n] = 0.0;
-        a.s.Fh[5 * plane + n] = 0.0;
-      }
-      if (d0 || d1) a.s.mask[n] = (uint8_t)((t0 > 0 ? 1 : 0) | (t1 > 0 ? 2 : 0));
-    }
-    __syncthreads();  // LDS lists are rewritten by the next brick
   }
+  const int total = listed ? m : a.n;
+  for (int base = 0; base < total; base += kMapChunk) {
+    const int cnt = min(kMapChunk, total - base);
+    if (tid < cnt) {
+      const int v = listed ? order[base + tid] : base + tid;
+      MapEntry& E = ent[tid];
+      for (int c = 0; c < 6; ++c) E.zone[c] = a.zone[6 * v + c];
+      E.tag = a.tag[v];
+      for (int c = 0; c < 4; ++c) E.xl[c] = a.xl[4 * v + c];
+      for (int c = 0; c < 6; ++c) E.kin[c] = a.kin[6 * v + c];
+      entid[tid] = v;
+    }
+    __syncthreads();
+    if (valid) {
+      for (int t = 0; t < cnt; ++t) {
+        const MapEntry& E = ent[t];
+        const int* z = E.zone;
+        if (i < z[0] || i >= z[1] || j < z[2] || j >= z[3] || k < z[4] || k >= z[5]) continue;
+        const int nj = z[3] - z[2], nk = z[5] - z[4];
+        const int tot = (z[1] - z[0]) * nj * nk;
+        double sf;
+        if (tot <= a.zone_cap) {
+          const int idx = ((i - z[0]) * nj + (j - z[2])) * nk + (k - z[4]);
+          const unsigned short cd = a.sfcode[(size_t)entid[t] * a.zone_cap + idx];
+          sf = cd == kSfCodeOne ? 1.0 : sf_from_count(g.dim, cd);
+        } else {
+          sf = solid_fraction(g.dim, a.hsq_half, ri, rj, rk, E.xl[0], E.xl[1], E.xl[2], E.xl[3]);
+        }
+        map_transition(a, E, sf, ri, rj, rk, n, S);
+      }
+    }
+    __syncthreads();
+  }
+  if (!valid) return;
+  if (S.d0) {
+    a.s.tag[n] = S.t0;
+    a.s.sf[n] = S.sf0;
+    a.s.us[n] = S.u0[0];
+    a.s.us[plane + n] = S.u0[1];
+    a.s.us[2 * plane + n] = S.u0[2];
+    a.s.Fh[n] = 0.0;
+    a.s.Fh[plane + n] = 0.0;
+    a.s.Fh[2 * plane + n] = 0.0;
+  }
+  if (S.d1) {
+    a.s.tag[plane + n] = S.t1;
+    a.s.sf[plane + n] = S.sf1;
+    a.s.us[3 * plane + n] = S.u1[0];
+    a.s.us[4 * plane + n] = S.u1[1];
+    a.s.us[5 * plane + n] = S.u1[2];
+    a.s.Fh[3 * plane + n] = 0.0;
+    a.s.Fh[4 * plane + n] = 0.0;
+    a.s.Fh[5 * plane + n] = 0.0;
+  }
+  if (S.d0 || S.d1) a.s.mask[n] = (uint8_t)((S.t0 > 0 ? 1 : 0) | (S.t1 > 0 ? 2 : 0));
 }
 
 // Force, torque and stresslet of one list entry; one workgroup per entry.
@@ -316,35 +402,55 @@ __global__ __launch_bounds__(256) void k_hydro(const HydroArgs a) {
   double acc[12];
   for (int c = 0; c < 12; ++c) acc[c] = 0.0;
   const int ni = z[1] - z[0], nj = z[3] - z[2], nk = z[5] - z[4];
-  const int64_t tot = (ni > 0 && nj > 0 && nk > 0) ? (int64_t)ni * nj * nk : 0;
-  for (int64_t t = threadIdx.x; t < tot; t += 256) {
-    const int lk = (int)(t % nk);
-    const int lj = (int)((t / nk) % nj);
-    const int li = (int)(t / ((int64_t)nk * nj));
-    const int i = z[0] + li, j = z[2] + lj, k = z[4] + lk;
-    const int n = (i * g.sy + j) * g.sz + k;
-    int s;
-    if (tag == a.slot_tag[n]) s = 0;
-    else if (tag == a.slot_tag[plane + n]) s = 1;
-    else continue;
-    const double F0 = a.Fh[(3 * s) * plane + n];
-    const double F1 = a.Fh[(3 * s + 1) * plane + n];
-    const double F2 = a.Fh[(3 * s + 2) * plane + n];
-    acc[0] += F0 * ff;
-    acc[1] += F1 * ff;
-    acc[2] += F2 * ff;
-    const double dx = (double)(i + g.off[0]) - xL;
-    const double dy = (double)(j + g.off[1]) - yL;
-    const double dz = (double)(k + g.off[2]) - zL;
-    acc[3] += (dy * F2 - dz * F1) * tf;
-    acc[4] += (dz * F0 - dx * F2) * tf;
-    acc[5] += (dx * F1 - dy * F0) * tf;
-    acc[6] -= 0.5 * (dx * F0 + dx * F0) * tf;
-    acc[7] -= 0.5 * (dy * F1 + dy * F1) * tf;
-    acc[8] -= 0.5 * (dz * F2 + dz * F2) * tf;
-    acc[9] -= 0.5 * (dy * F0 + dx * F1) * tf;
-    acc[10] -= 0.5 * (dz * F0 + dx * F2) * tf;
-    acc[11] -= 0.5 * (dz * F1 + dy * F2) * tf;
+  const int tot = (ni > 0 && nj > 0 && nk > 0) ? ni * nj * nk : 0;
+  const int njk = nj * nk;
+  // U nodes per thread per pass so that their loads are in flight together;
+  // 32-bit index arithmetic (a zone holds far fewer than 2^31 nodes)
+  constexpr int U = 8;
+  for (int base = threadIdx.x; base < tot; base += 256 * U) {
+    int nn[U], sl[U], ii[U], jj[U], kk[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = base + 256 * u;
+      sl[u] = -1;
+      nn[u] = 0;
+      ii[u] = jj[u] = kk[u] = 0;
+      if (t < tot) {
+        const int li = t / njk;
+        const int rem = t - li * njk;
+        const int lj = rem / nk;
+        const int lk = rem - lj * nk;
+        ii[u] = z[0] + li;
+        jj[u] = z[2] + lj;
+        kk[u] = z[4] + lk;
+        nn[u] = (ii[u] * g.sy + jj[u]) * g.sz + kk[u];
+        const int a0 = a.slot_tag[nn[u]], a1 = a.slot_tag[plane + nn[u]];
+        sl[u] = tag == a0 ? 0 : (tag == a1 ? 1 : -1);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (sl[u] < 0) continue;
+      const int s = sl[u], n = nn[u];
+      const double F0 = a.Fh[(3 * s) * plane + n];
+      const double F1 = a.Fh[(3 * s + 1) * plane + n];
+      const double F2 = a.Fh[(3 * s + 2) * plane + n];
+      acc[0] += F0 * ff;
+      acc[1] += F1 * ff;
+      acc[2] += F2 * ff;
+      const double dx = (double)(ii[u] + g.off[0]) - xL;
+      const double dy = (double)(jj[u] + g.off[1]) - yL;
+      const double dz = (double)(kk[u] + g.off[2]) - zL;
+      acc[3] += (dy * F2 - dz * F1) * tf;
+      acc[4] += (dz * F0 - dx * F2) * tf;
+      acc[5] += (dx * F1 - dy * F0) * tf;
+      acc[6] -= 0.5 * (dx * F0 + dx * F0) * tf;
+      acc[7] -= 0.5 * (dy * F1 + dy * F1) * tf;
+      acc[8] -= 0.5 * (dz * F2 + dz * F2) * tf;
+      acc[9] -= 0.5 * (dy * F0 + dx * F1) * tf;
+      acc[10] -= 0.5 * (dz * F0 + dx * F2) * tf;
+      acc[11] -= 0.5 * (dz * F1 + dy * F2) * tf;
+    }
   }
   __shared__ double red[12][256];
   for (int c = 0; c < 12; ++c) red[c][threadIdx.x] = acc[c];
@@ -421,32 +527,22 @@ hipError_t launch_fill_int(int* p, int64_t n, int value, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_prep_count(const PrepArgs& a, hipStream_t s) {
+hipError_t launch_prep_bin(const PrepArgs& a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_prep_count, grid_for(a.n, 256), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_prep_bin, dim3(a.n), dim3(64), 0, s, a);
   return hipGetLastError();
 }
 
-hipError_t launch_scan_bricks(BrickDev& b, hipStream_t s) {
-  const int64_t n = b.nbricks;
-  const int nb = (int)((n + 1023) / 1024);
-  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, s, b.nonempty_n, (int64_t)1, 0);
-  hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(1024), 0, s, b.count, b.offset, b.block_sums, n);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, b.block_sums, nb);
-  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, s, b.count, b.offset, b.block_sums,
-                     b.cursor, b.nonempty, b.nonempty_n, n);
+hipError_t launch_entry_sf(const MapArgs& a, hipStream_t s) {
+  if (a.n <= 0 || a.zone_cap <= 0) return hipSuccess;
+  const int64_t chunks = (a.zone_cap + kSfChunk - 1) / kSfChunk;
+  hipLaunchKernelGGL(k_entry_sf, dim3((unsigned)(a.n * chunks)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
-hipError_t launch_fill_lists(const PrepArgs& a, const BrickDev& b, hipStream_t s) {
-  if (a.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_lists, grid_for(a.n, 256), dim3(256), 0, s, a, b.offset, b.cursor,
-                     b.list);
-  return hipGetLastError();
-}
-
-hipError_t launch_map(const MapArgs& a, int* overflow, hipStream_t s, int grid) {
-  hipLaunchKernelGGL(k_map, dim3(grid), dim3(256), 0, s, a, overflow);
+hipError_t launch_map(const MapArgs& a, int64_t nbricks, hipStream_t s) {
+  if (nbricks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_map, dim3((unsigned)nbricks), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
