@@ -202,3 +202,19 @@ def test_zero_particles_and_errors():
         lat.set_bc(99, 0.0)
     with pytest.raises(lbmpsm.LbmPsmError):
         lbmpsm.Lattice(lbmpsm.lattice_units_config(3, (8, 8, 8), (1, 1, 1), 0.5))
+
+
+@pytest.mark.parametrize("per", [(1, 0, 1), (0, 0, 1)])
+def test_moving_sphere_wall_axis_without_bc_long(per):
+    """The Stokes-drag deck's setting at reduced size: `boundary p f p` and no
+    lbm-psm-bc, so the y halo layers evolve on their own (stale values,
+    src/LBM_PSM_MPICOMM.h:129,140), with a sphere crossing the periodic x
+    boundary; 600 steps."""
+    n = (30, 31, 30)
+    box = lattice_box(3, n, per)
+    pos = np.array([[27.5, 15.0, 15.0]])
+    atoms = random_atoms(box, [3.0], seed=9, positions=pos, vmax=0.0, wmax=0.0)
+    atoms.v[:] = [[0.01, 0.0, 0.0]]
+    lat, ora, out = run_pair(3, n, per, 0.7, (0, 0, 0), "none", atoms, 600, cutoff=8.0,
+                             label="stokes-like")
+    compare(lat, ora, out)

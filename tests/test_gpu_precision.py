@@ -1,0 +1,63 @@
+"""FP32 / FP16 population storage against the FP64 path on the same inputs.
+
+FP32 stores f - w*rho0 and computes in FP32; FP16 stores the same deviations
+scaled by 2^12 and computes in FP32; node forces stay per node, per-particle
+force/torque/stresslet and the mass are summed in FP64.  The FP64 path is
+the parity form (checked against the oracle in test_gpu_parity.py); the
+error of each reduced-precision result is measured against it.
+
+Bars: FP32 drag and torque within 1e-5 (relative to the largest particle
+force / torque), velocity field within 1e-5 of the largest velocity, mass
+drift below 1e-6 per node over the run.  FP16 is reported with bars ten
+times looser than its storage rounding (2^-11 of the stored deviations).
+"""
+import numpy as np
+import pytest
+
+import lbmpsm
+from lbmpsm import harness as H
+from scenarios import lattice_box, random_atoms
+
+pytestmark = pytest.mark.gpu
+
+
+def run(storage, steps=300):
+    n = (64, 64, 64)
+    per = (1, 0, 1)
+    box = lattice_box(3, n, per)
+    atoms = random_atoms(box, [8.0] * 12, seed=21, vmax=1e-3, wmax=1e-4, gap=2.0)
+    lat = lbmpsm.Lattice(lbmpsm.lattice_units_config(3, n, per, 0.7, storage=storage,
+                                                     F_lbm=(2e-6, 0, 0)))
+    lat.set_bc("shear")
+    for s in range(steps):
+        H.move_linear(atoms, box, s, 1.0)
+        lst = H.build_list(atoms, box, 12.0)
+        lat.set_particles(lst.tag, lst.x, lst.v, lst.omega, lst.radius)
+        lat.step(store_macros=(s == steps - 1))
+        F, T, S = lat.hydro()
+    rho, u = lat.macros()
+    m = lat.mass()
+    lat.close()
+    return F, T, S, u, m
+
+
+@pytest.fixture(scope="module")
+def ref64():
+    print("\n[case] precision reference FP64 64^3 shear 12 spheres 300 steps", flush=True)
+    return run("f64")
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f16", 2e-2)])
+def test_reduced_storage_error(ref64, storage, tol):
+    print(f"\n[case] precision {storage}", flush=True)
+    F, T, S, u, m = run(storage)
+    F0, T0, S0, u0, m0 = ref64
+    eF = np.max(np.linalg.norm(F - F0, axis=1)) / np.max(np.linalg.norm(F0, axis=1))
+    eT = np.max(np.linalg.norm(T - T0, axis=1)) / np.max(np.linalg.norm(T0, axis=1))
+    eS = np.max(np.abs(S - S0)) / np.max(np.abs(S0))
+    eu = np.max(np.abs(u - u0)) / np.max(np.abs(u0))
+    em = abs(m - m0) / (64 * 64 * 64)
+    print(f"  {storage}: drag {eF:.2e} torque {eT:.2e} stresslet {eS:.2e} velocity {eu:.2e} "
+          f"mass/node {em:.2e}", flush=True)
+    assert eF <= tol and eT <= tol and eu <= tol
+    assert em <= (1e-6 if storage == "f32" else 1e-4)
