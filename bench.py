@@ -84,7 +84,7 @@ def run_gpu(storage, lists, warmup, steps):
     lat.sync()
     t1 = time.perf_counter()
     prof = {k: lat.profile_read(k) for k in ("collide_stream", "collide_faces", "map_bin",
-                                              "map_particles", "hydro_reduce")}
+                                              "map_fraction", "map_particles", "hydro_reduce")}
     lat.profile(False)
     info = lat.info
     lat.close()
@@ -153,6 +153,18 @@ def main():
                       f"k_collide_lowp<3,19,{ {'f32': 1, 'f16': 2}[args.storage]}>",
             "bytes_per_update": bytes_per_update, "nodes_per_launch": int(bulk_nodes),
             "avg_launch_ms": round(avg_ms, 5)}
+    # HBM bytes per launch of the dominant kernel from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy kernel)
+    tpath = os.path.join(ROOT, "profiles", f"collide_{args.storage}_192cube_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tr = json.load(fh)
+        roof["traffic"] = tr.get("hbm_bytes_per_launch")
+        roof["traffic_source"] = os.path.relpath(tpath, ROOT)
+    from lbmpsm import probe
+    copy_gbs, _ = probe.copy_bandwidth()
+    roof["copy_ceiling"] = round(copy_gbs, 1)
+    roof["frac_of_copy"] = round(achieved / copy_gbs, 4)
     kernels = {k: {"avg_ms": round(v[0] / max(v[1], 1), 5), "launches": v[1]}
                for k, v in prof.items()}
 
