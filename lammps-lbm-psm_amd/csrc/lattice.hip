@@ -15,6 +15,8 @@
 // contraction enabled locally.
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+
 #include "lbmpsm_internal.h"
 
 // Tuning knobs of the streaming kernels (compile-time).
@@ -31,6 +33,13 @@
 #define LBMPSM_OCC __attribute__((amdgpu_waves_per_eu(LBMPSM_WAVES_PER_EU)))
 #else
 #define LBMPSM_OCC
+#endif
+#ifndef LBMPSM_ROW_WAVES
+#define LBMPSM_ROW_WAVES 1  // waves per bulk workgroup; 0: the chunks of one row, at most 4
+#endif
+static_assert(LBMPSM_ROW_WAVES >= 0 && LBMPSM_ROW_WAVES <= 4, "bulk workgroups hold at most 4 waves");
+#ifndef LBMPSM_XCD_SWIZZLE
+#define LBMPSM_XCD_SWIZZLE 0  // 1: give each XCD a contiguous range of workgroups
 #endif
 
 namespace lbmpsm {
@@ -247,6 +256,21 @@ __device__ __forceinline__ bool on_face(const Face* faces, int nfaces, int i, in
 // FP64 collision, reference expression order
 // (src/LBM_PSM_BGK_dynamics.h:40-114, src/LBM_PSM_dynamics.h:43-116).
 // ---------------------------------------------------------------------------
+// Bulk kernels run one wave per 64-node chunk of a row; a workgroup holds
+// several consecutive waves, normally the chunks of one row, so that the
+// cache lines a z-shifted population straddles between chunks are fetched
+// once per compute unit.
+__device__ __forceinline__ int bulk_wave() {
+  int b = (int)blockIdx.x;
+#if LBMPSM_XCD_SWIZZLE
+  // workgroups are dealt round-robin over the 8 XCDs; renumber them so
+  // that each XCD (and its L2) walks one contiguous range of rows
+  const int nb = (int)gridDim.x, per = nb >> 3, rem = nb & 7, x = b & 7;
+  b = x * per + min(x, rem) + (b >> 3);
+#endif
+  return b * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+}
+
 // Collision of one node whose post-stream populations are in f[] (FP64).
 template <int DIM, int Q>
 __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, unsigned rb,
@@ -254,11 +278,11 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
 
 // Bulk: every node that is on no Zou-He face.
 template <int DIM, int Q>
-__global__ __launch_bounds__(64) LBMPSM_OCC void k_collide_f64(const CollideArgs a) {
+__global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_f64(const CollideArgs a) {
   const Geom& g = a.g;
   Row r;
-  if (!make_row<DIM>(g, blockIdx.x, r)) return;
-  const int c = r.c0 + (int)threadIdx.x;
+  if (!make_row<DIM>(g, bulk_wave(), r)) return;
+  const int c = r.c0 + (int)(threadIdx.x & 63);
   if (c >= r.ncols) return;
   if (!a.poststream && a.nfaces &&
       on_face(a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c, DIM == 3 ? c : 0, kMaxFaces))
@@ -412,11 +436,11 @@ __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, uns
                                              unsigned uc);
 
 template <int DIM, int Q, int ST>
-__global__ __launch_bounds__(64) LBMPSM_OCC void k_collide_lowp(const CollideArgs a) {
+__global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_lowp(const CollideArgs a) {
   const Geom& g = a.g;
   Row r;
-  if (!make_row<DIM>(g, blockIdx.x, r)) return;
-  const int c = r.c0 + (int)threadIdx.x;
+  if (!make_row<DIM>(g, bulk_wave(), r)) return;
+  const int c = r.c0 + (int)(threadIdx.x & 63);
   if (c >= r.ncols) return;
   if (!a.poststream && a.nfaces &&
       on_face(a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c, DIM == 3 ? c : 0, kMaxFaces))
@@ -674,11 +698,13 @@ template <int DIM, int Q>
 static hipError_t collide_dim(int storage, const CollideArgs& a, hipStream_t s) {
   const int64_t rows = rows_of(a.g);
   const int ncols = DIM == 3 ? a.g.sz : a.g.sy;
-  dim3 grid((unsigned)(rows * ((ncols + 63) / 64)));
+  const int64_t waves = rows * ((ncols + 63) / 64);
+  const int wpb = LBMPSM_ROW_WAVES > 0 ? LBMPSM_ROW_WAVES : std::min((ncols + 63) / 64, 4);
+  dim3 grid((unsigned)((waves + wpb - 1) / wpb)), block((unsigned)(64 * wpb));
   switch (storage) {
-    case ST_F64: hipLaunchKernelGGL((k_collide_f64<DIM, Q>), grid, dim3(64), 0, s, a); break;
-    case ST_F32: hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32>), grid, dim3(64), 0, s, a); break;
-    case ST_F16: hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16>), grid, dim3(64), 0, s, a); break;
+    case ST_F64: hipLaunchKernelGGL((k_collide_f64<DIM, Q>), grid, block, 0, s, a); break;
+    case ST_F32: hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32>), grid, block, 0, s, a); break;
+    case ST_F16: hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16>), grid, block, 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
