@@ -168,6 +168,36 @@ int lbmpsm_profile_enable(lbmpsm_ctx* ctx, int enable);
 int lbmpsm_profile_read(lbmpsm_ctx* ctx, const char* kernel, double* total_ms, int64_t* count);
 int lbmpsm_profile_reset(lbmpsm_ctx* ctx);
 
+/* ------------------------------------------------------------------------
+ * Adjoint (no counterpart in the reference): gradients of a loss on the
+ * per-entry force, torque and stresslet after T steps, with the particles
+ * frozen (same list, positions and velocities every step), on an FP64,
+ * fully periodic context without `lbm-psm-bc` faces.
+ *
+ * lbmpsm_adjoint_forward runs T = `steps` LBM steps from the context's
+ * current state, mapping the list before every step as lbmpsm_set_particles
+ * does, and keeps an FP64 copy of the populations every `checkpoint_every`
+ * steps.  F, T, S (any may be NULL) receive the last step's per-entry
+ * results, as lbmpsm_get_hydro.  Memory: ceil(T/k) + k - 1 population
+ * copies.
+ *
+ * lbmpsm_adjoint_backward takes dLoss/dF [n][3], dLoss/dT [n][3] and
+ * dLoss/dS [n][6] of the last step (NULL = zero) and returns
+ *   grad_F_lbm[3]   d/d cfg.F_lbm (lattice-unit body force),
+ *   grad_tau[1]     d/d cfg.tau (unit conversion factors held fixed),
+ *   grad_v[n][3]    d/d v of each list entry (physical units, as passed),
+ *   grad_omega[n][3] d/d omega of each list entry.
+ * It recomputes each segment from its checkpoint and leaves the populations
+ * in the forward's final state.  Call it after lbmpsm_adjoint_forward with
+ * no other call on the context in between. */
+int lbmpsm_adjoint_forward(lbmpsm_ctx* ctx, int steps, int checkpoint_every, int n,
+                           const int* tag, const double* x, const double* v,
+                           const double* omega, const double* radius, double* F, double* T,
+                           double* S);
+int lbmpsm_adjoint_backward(lbmpsm_ctx* ctx, int n, const double* Fbar, const double* Tbar,
+                            const double* Sbar, double* grad_F_lbm, double* grad_tau,
+                            double* grad_v, double* grad_omega);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,23 @@ struct lbmpsm_ctx {
 
   int64_t device_bytes = 0;
 
+  // adjoint of a run with frozen particles (lbmpsm_adjoint_*)
+  struct Adj {
+    int ready = 0;
+    int steps = 0, every = 0, ncp = 0, n = 0;
+    std::vector<int> post;          // per checkpoint: 1 = post-stream state
+    double* ckpt = nullptr;         // [ncp][q*plane] state at each segment start
+    double* seg = nullptr;          // [every-1][q*plane] states inside a segment
+    double* lam = nullptr;          // [2][q*plane]
+    double* acc = nullptr;          // [10][plane]
+    double* seed = nullptr;         // [6][plane]
+    int* writer = nullptr;          // [2][plane]
+    double* bar = nullptr;          // [n][12]
+    double* eout = nullptr;         // [n][6]
+    double* partials = nullptr;     // [1024][4]
+    size_t ckpt_n = 0, seg_n = 0, ent_n = 0;
+  } adj;
+
   // profiling
   int prof = 0;
   std::vector<ProfEntry> pending;
@@ -613,6 +630,11 @@ int lbmpsm_destroy(lbmpsm_ctx* c) {
   dev_free(c->sfcode);
   dev_free(c->d_cb);
   dev_free(c->d_faces);
+  {
+    auto& A = c->adj;
+    void* aa[] = {A.ckpt, A.seg, A.lam, A.acc, A.seed, A.writer, A.bar, A.eout, A.partials};
+    for (void* p : aa) dev_free(p);
+  }
   if (c->h_stage) (void)hipHostFree(c->h_stage);
   if (c->h_out) (void)hipHostFree(c->h_out);
   for (auto& p : c->pending) {
@@ -900,6 +922,223 @@ int lbmpsm_profile_reset(lbmpsm_ctx* c) {
   int rc = prof_collect(c);
   c->totals.clear();
   return rc;
+}
+
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Adjoint of a run with frozen particles (FP64, fully periodic, no faces).
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+int adj_grow(lbmpsm_ctx* c, T** p, size_t& have, size_t need) {
+  if (*p && have >= need) return LBMPSM_OK;
+  if (*p) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dev_free(*p);
+    *p = nullptr;
+    c->device_bytes -= (int64_t)(have * sizeof(T));
+    have = 0;
+  }
+  int rc = dev_alloc(c, p, need);
+  if (rc) return rc;
+  have = need;
+  return LBMPSM_OK;
+}
+
+int adj_supported(lbmpsm_ctx* c) {
+  if (c->cfg.storage != LBMPSM_STORE_F64)
+    return fail(c, LBMPSM_ERR_STATE, "the adjoint runs on FP64 storage only");
+  if (c->nfaces)
+    return fail(c, LBMPSM_ERR_STATE, "the adjoint supports lattices without Zou-He faces");
+  for (int d = 0; d < c->g.dim; ++d)
+    if (!c->g.per[d])
+      return fail(c, LBMPSM_ERR_STATE, "the adjoint supports fully periodic lattices only");
+  return LBMPSM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbmpsm_adjoint_forward(lbmpsm_ctx* c, int steps, int checkpoint_every, int n,
+                           const int* tag, const double* x, const double* v,
+                           const double* omega, const double* radius, double* F, double* T,
+                           double* S) {
+  if (!c) return LBMPSM_ERR_ARG;
+  if (steps < 1 || checkpoint_every < 1)
+    return fail(c, LBMPSM_ERR_ARG, "steps and checkpoint_every must be >= 1");
+  int rc = adj_supported(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  auto& A = c->adj;
+  A.ready = 0;
+  const size_t S1 = (size_t)c->g.q * (size_t)c->g.plane;
+  const int ncp = (steps + checkpoint_every - 1) / checkpoint_every;
+  rc = adj_grow(c, &A.ckpt, A.ckpt_n, (size_t)ncp * S1);
+  if (rc) return rc;
+  A.post.assign(ncp, 0);
+  HIPCHK(c, hipMemcpyAsync(A.ckpt, c->f[c->cur], S1 * sizeof(double), hipMemcpyDeviceToDevice,
+                           c->stream));
+  A.post[0] = c->poststream;
+  for (int t = 1; t <= steps; ++t) {
+    rc = lbmpsm_set_particles(c, n, tag, x, v, omega, radius);
+    if (rc) return rc;
+    rc = lbmpsm_step(c, 0);
+    if (rc) return rc;
+    if (t < steps && t % checkpoint_every == 0) {
+      const int k = t / checkpoint_every;
+      HIPCHK(c, hipMemcpyAsync(A.ckpt + (size_t)k * S1, c->f[c->cur], S1 * sizeof(double),
+                               hipMemcpyDeviceToDevice, c->stream));
+      A.post[k] = c->poststream;
+    }
+  }
+  if (F || T || S) {
+    rc = lbmpsm_get_hydro(c, n, F, T, S);
+    if (rc) return rc;
+  }
+  A.steps = steps;
+  A.every = checkpoint_every;
+  A.ncp = ncp;
+  A.n = n;
+  A.ready = 1;
+  return LBMPSM_OK;
+}
+
+int lbmpsm_adjoint_backward(lbmpsm_ctx* c, int n, const double* Fbar, const double* Tbar,
+                            const double* Sbar, double* grad_F_lbm, double* grad_tau,
+                            double* grad_v, double* grad_omega) {
+  if (!c) return LBMPSM_ERR_ARG;
+  auto& A = c->adj;
+  if (!A.ready) return fail(c, LBMPSM_ERR_STATE, "lbmpsm_adjoint_forward has not run");
+  if (n != A.n || n != c->part.n)
+    return fail(c, LBMPSM_ERR_ARG, "n differs from the particle count of the forward run");
+  int rc = adj_supported(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  const Geom& g = c->g;
+  const size_t P = (size_t)g.plane;
+  const size_t S1 = (size_t)g.q * P;
+  rc = adj_grow(c, &A.seg, A.seg_n, (size_t)std::max(A.every - 1, 1) * S1);
+  if (rc) return rc;
+  if (!A.lam) {  // plane-sized buffers: allocated once per context
+    rc = dev_alloc(c, &A.lam, 2 * S1);
+    rc |= dev_alloc(c, &A.acc, 10 * P);
+    rc |= dev_alloc(c, &A.seed, 6 * P);
+    rc |= dev_alloc(c, &A.writer, 2 * P);
+    rc |= dev_alloc(c, &A.partials, 1024 * 4);
+    if (rc) return LBMPSM_ERR_NOMEM;
+  }
+  if (A.ent_n < (size_t)std::max(n, 1)) {  // per-entry buffers grow with n
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dev_free(A.bar);
+    dev_free(A.eout);
+    A.bar = A.eout = nullptr;
+    c->device_bytes -= (int64_t)(A.ent_n * 18 * sizeof(double));
+    A.ent_n = 0;
+    rc = dev_alloc(c, &A.bar, (size_t)std::max(n, 1) * 12);
+    rc |= dev_alloc(c, &A.eout, (size_t)std::max(n, 1) * 6);
+    if (rc) return LBMPSM_ERR_NOMEM;
+    A.ent_n = (size_t)std::max(n, 1);
+  }
+
+  // per-entry loss gradients
+  std::vector<double> hb((size_t)std::max(n, 1) * 12, 0.0);
+  for (int e = 0; e < n; ++e)
+    for (int d = 0; d < 3; ++d) {
+      if (Fbar) hb[12 * e + d] = Fbar[3 * e + d];
+      if (Tbar) hb[12 * e + 3 + d] = Tbar[3 * e + d];
+    }
+  if (Sbar)
+    for (int e = 0; e < n; ++e)
+      for (int c6 = 0; c6 < 6; ++c6) hb[12 * e + 6 + c6] = Sbar[6 * e + c6];
+  HIPCHK(c, hipMemcpyAsync(A.bar, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+  AdjSeedArgs sa{};
+  sa.g = g;
+  sa.n = n;
+  sa.tag = c->part.tag(c->part.n);
+  sa.xl = c->part.xl;
+  sa.zone = c->part.zone;
+  sa.rzone = c->part.rzone;
+  sa.slot_tag = c->slots.tag;
+  sa.mask = c->slots.mask;
+  sa.bar = A.bar;
+  sa.seed = A.seed;
+  sa.writer = A.writer;
+  sa.forceFactor = c->cfg.force_factor;
+  sa.torqueFactor = c->cfg.torque_factor;
+  HIPCHK(c, launch_adj_seed(sa, c->stream));
+  HIPCHK(c, launch_adj_writer(sa, c->stream));
+  HIPCHK(c, hipMemsetAsync(A.acc, 0, 10 * P * sizeof(double), c->stream));
+
+  CollideArgs base = make_args(c, 0);
+  AdjArgs aa{};
+  aa.g = g;
+  aa.mask = c->slots.mask;
+  aa.sf = c->slots.sf;
+  aa.us = c->slots.us;
+  aa.acc = A.acc;
+  aa.omega = base.omega;
+  aa.invCs2 = base.invCs2;
+  aa.invCs4 = base.invCs4;
+  for (int d = 0; d < 3; ++d) aa.F[d] = base.F[d];
+  double* lam[2] = {A.lam, A.lam + S1};
+  int lc = 0;
+  bool have_lam = false;
+  for (int sg = A.ncp - 1; sg >= 0; --sg) {
+    const int t0 = sg * A.every;
+    const int t1 = std::min(t0 + A.every, A.steps);
+    const int L = t1 - t0;
+    // X[0] = checkpoint, X[i] = post-collision state after step t0 + i
+    auto X = [&](int i) -> double* {
+      return i == 0 ? A.ckpt + (size_t)sg * S1 : A.seg + (size_t)(i - 1) * S1;
+    };
+    for (int i = 1; i < L; ++i) {
+      CollideArgs a = base;
+      a.fin = X(i - 1);
+      a.fout = X(i);
+      a.poststream = i == 1 ? A.post[sg] : 0;
+      HIPCHK(c, launch_collide(LBMPSM_STORE_F64, a, c->stream));
+    }
+    for (int t = t1; t > t0; --t) {
+      const int i = t - t0;
+      aa.hprev = X(i - 1);
+      aa.inplace = i == 1 ? A.post[sg] : 0;
+      aa.lam_in = have_lam ? lam[lc] : nullptr;
+      aa.lam_out = lam[1 - lc];
+      aa.seed = t == A.steps ? A.seed : nullptr;
+      HIPCHK(c, launch_adj_step(aa, c->stream));
+      lc = 1 - lc;
+      have_lam = true;
+    }
+  }
+  HIPCHK(c, launch_adj_entry(sa, A.acc, A.eout, c->stream));
+  const int nb = (int)std::min<int64_t>(1024, (g.nodes + 255) / 256);
+  HIPCHK(c, launch_adj_param(g, A.acc, A.partials, nb, c->stream));
+  std::vector<double> eo((size_t)std::max(n, 1) * 6), pp((size_t)nb * 4);
+  if (n > 0)
+    HIPCHK(c, hipMemcpyAsync(eo.data(), A.eout, (size_t)n * 6 * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(pp.data(), A.partials, pp.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double gp[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 4; ++k) gp[k] += pp[4 * b + k];
+  // om = 1/tau; v_lb = v / Uc * u_lb; w_lb = w * tc * dt_d
+  if (grad_tau) *grad_tau = gp[0] * (-1.0 / (c->cfg.tau * c->cfg.tau));
+  if (grad_F_lbm)
+    for (int d = 0; d < 3; ++d) grad_F_lbm[d] = gp[1 + d];
+  const double sv = c->cfg.u_lb / c->cfg.Uc, sw = c->cfg.tc * c->cfg.dt_d;
+  for (int e = 0; e < n; ++e)
+    for (int d = 0; d < 3; ++d) {
+      if (grad_v) grad_v[3 * e + d] = eo[6 * e + d] * sv;
+      if (grad_omega) grad_omega[3 * e + d] = eo[6 * e + 3 + d] * sw;
+    }
+  return LBMPSM_OK;
 }
 
 }  // extern "C"

@@ -175,4 +175,46 @@ hipError_t launch_macros_export(const Geom& g, const double* rho, const double* 
                                 double* rho_out, double* u_out, hipStream_t s);
 hipError_t launch_slots_reset(const Geom& g, const SlotsDev& sl, hipStream_t s);
 
+// ---- adjoint.hip -------------------------------------------------------------
+// One reverse step of the FP64 step on a fully periodic lattice without
+// faces: lambda^t = J^T (pull^T lambda^{t+1}) plus the node-force seed on the
+// last step; parameter gradients accumulate into acc[10][plane]
+// (om, F[3], u_s slot 0 [3], u_s slot 1 [3]).
+struct AdjArgs {
+  Geom g;
+  const double* hprev;    // [q][plane] state before the step (post-collision of t-1)
+  int inplace;            // 1: hprev holds post-stream values (read at the node)
+  const double* lam_in;   // [q][plane] adjoint of the next step's input, or null
+  double* lam_out;        // [q][plane]
+  const uint8_t* mask;
+  const double* sf;       // [2][plane]
+  const double* us;       // [6][plane]
+  const double* seed;     // [6][plane] d loss / d node force of this step, or null
+  double* acc;            // [10][plane]
+  double omega, invCs2, invCs4;
+  double F[3];
+};
+struct AdjSeedArgs {
+  Geom g;
+  int n;
+  const int* tag;
+  const double* xl;
+  const int* zone;
+  const int* rzone;
+  const int* slot_tag;   // [2][plane]
+  const uint8_t* mask;
+  const double* bar;     // [n][12] d loss / d (F, T, S) per entry
+  double* seed;          // [6][plane]
+  int* writer;           // [2][plane]
+  double forceFactor, torqueFactor;
+};
+hipError_t launch_adj_step(const AdjArgs& a, hipStream_t s);
+hipError_t launch_adj_seed(const AdjSeedArgs& a, hipStream_t s);
+hipError_t launch_adj_writer(const AdjSeedArgs& a, hipStream_t s);
+// out[n][6]: d/dv_lb, d/dw_lb per entry
+hipError_t launch_adj_entry(const AdjSeedArgs& a, const double* acc, double* out, hipStream_t s);
+// partials[nblocks][4]: per-block sums of acc planes 0..3
+hipError_t launch_adj_param(const Geom& g, const double* acc, double* partials, int nblocks,
+                            hipStream_t s);
+
 }  // namespace lbmpsm

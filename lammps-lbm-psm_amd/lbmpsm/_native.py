@@ -115,6 +115,9 @@ def _load() -> C.CDLL:
         "lbmpsm_profile_enable": (C.c_int, [P, C.c_int]),
         "lbmpsm_profile_read": (C.c_int, [P, C.c_char_p, dp, C.POINTER(C.c_int64)]),
         "lbmpsm_profile_reset": (C.c_int, [P]),
+        "lbmpsm_adjoint_forward": (C.c_int, [P, C.c_int, C.c_int, C.c_int, ip, dp, dp, dp, dp,
+                                             dp, dp, dp]),
+        "lbmpsm_adjoint_backward": (C.c_int, [P, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

@@ -130,6 +130,45 @@ class Lattice:
         self._check(N.LIB.lbmpsm_get_hydro(self._h, n, N.dptr(F), N.dptr(T), N.dptr(S)))
         return F, T, S
 
+    # -- adjoint (frozen particles, FP64, fully periodic, no faces) ------------
+    def adjoint_forward(self, steps, checkpoint_every, tag, x, v, omega, radius):
+        """`steps` LBM steps with the same particle list mapped before each;
+        returns the last step's per-entry F, T, S and keeps checkpoints for
+        adjoint_backward."""
+        tag = np.ascontiguousarray(tag, dtype=np.int32)
+        n = tag.shape[0]
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(n, 3)
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(n, 3)
+        w = np.ascontiguousarray(omega, dtype=np.float64).reshape(n, 3)
+        r = np.ascontiguousarray(radius, dtype=np.float64).reshape(n)
+        F = np.zeros((n, 3))
+        T = np.zeros((n, 3))
+        S = np.zeros((n, 6))
+        self._check(N.LIB.lbmpsm_adjoint_forward(
+            self._h, int(steps), int(checkpoint_every), n, N.iptr(tag), N.dptr(x), N.dptr(v),
+            N.dptr(w), N.dptr(r), N.dptr(F), N.dptr(T), N.dptr(S)))
+        self._n = n
+        return F, T, S
+
+    def adjoint_backward(self, Fbar=None, Tbar=None, Sbar=None):
+        """Gradients of a loss whose derivatives with respect to the last
+        step's F, T, S are given: (d/dF_lbm [3], d/dtau, d/dv [n,3],
+        d/domega [n,3])."""
+        n = self._n
+
+        def arr(a, cols):
+            return None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(n, cols)
+        Fb, Tb, Sb = arr(Fbar, 3), arr(Tbar, 3), arr(Sbar, 6)
+        gF = np.zeros(3)
+        gtau = C.c_double()
+        gv = np.zeros((n, 3))
+        gw = np.zeros((n, 3))
+        self._check(N.LIB.lbmpsm_adjoint_backward(
+            self._h, n, None if Fb is None else N.dptr(Fb), None if Tb is None else N.dptr(Tb),
+            None if Sb is None else N.dptr(Sb), N.dptr(gF), C.byref(gtau), N.dptr(gv),
+            N.dptr(gw)))
+        return gF, gtau.value, gv, gw
+
     # -- read-back ------------------------------------------------------------
     def populations(self):
         nx, ny, nz = self.shape
