@@ -122,6 +122,17 @@ int lbmpsm_set_bc(lbmpsm_ctx* ctx, int keyword, double u_infty, double rho_outle
 int lbmpsm_set_particles(lbmpsm_ctx* ctx, int n, const int* tag, const double* x,
                          const double* v, const double* omega, const double* radius);
 
+/* The same as lbmpsm_set_particles for a list that is already in device
+ * memory on the context's device: tag, x, v, omega and radius are device
+ * pointers with the same layouts, and max_radius bounds the radii (it sizes
+ * the per-entry solid-fraction tables; an entry above it is still mapped
+ * exactly, by the slower in-kernel path).  No host transfer and no
+ * synchronisation.  No counterpart in the reference: for callers whose
+ * particle data live on the GPU. */
+int lbmpsm_set_particles_device(lbmpsm_ctx* ctx, int n, const int* tag, const double* x,
+                                const double* v, const double* omega, const double* radius,
+                                double max_radius);
+
 /* One LBM step: collide-stream of every node with PSM weighting and Guo
  * forcing, the selected Zou-He faces, and the per-particle force, torque and
  * stresslet reduction for the list last given to lbmpsm_set_particles.
@@ -161,7 +172,8 @@ int lbmpsm_mass(lbmpsm_ctx* ctx, double* mass);
 
 int lbmpsm_sync(lbmpsm_ctx* ctx);
 
-/* Per-kernel timing with HIP events on the context's stream. */
+/* Per-kernel timing with HIP events on the context's stream: enable 0 off,
+ * 1 every kernel, 2 the bulk collide-stream only (two events per step). */
 int lbmpsm_profile_enable(lbmpsm_ctx* ctx, int enable);
 /* Total milliseconds and launch count recorded for a kernel name
  * ("collide_stream", "map_particles", "hydro_reduce", ...). */

@@ -132,15 +132,18 @@ struct ProfScope {
   lbmpsm_ctx* c;
   const char* name;
   hipEvent_t a = nullptr, b = nullptr;
+  bool on = false;
   ProfScope(lbmpsm_ctx* c_, const char* n) : c(c_), name(n) {
-    if (c->prof) {
+    // mode 1: every kernel; mode 2: the bulk collide only
+    on = c->prof == 1 || (c->prof == 2 && std::strcmp(n, "collide_stream") == 0);
+    if (on) {
       a = take_event(c);
       b = take_event(c);
       if (a && b) (void)hipEventRecord(a, c->stream);
     }
   }
   ~ProfScope() {
-    if (c->prof && a && b) {
+    if (on && a && b) {
       (void)hipEventRecord(b, c->stream);
       c->pending.push_back({name, a, b});
     }
@@ -298,6 +301,33 @@ int ensure_particles(lbmpsm_ctx* c, int n) {
   return LBMPSM_OK;
 }
 
+// Codes per entry for k_entry_sf: the largest zone box, (2 r/dx + 8) nodes
+// per axis bounds every map zone (src/LBM_PSM_exchangeParticleData.cpp:41-48).
+int size_codes(lbmpsm_ctx* c, int n, double rmax) {
+  int rc = 0;
+  const double span = 2.0 * rmax / c->cfg.dx + 8.0;
+  int64_t ext[3];
+  const int se[3] = {c->g.sx, c->g.sy, c->g.sz};
+  for (int d = 0; d < 3; ++d) ext[d] = std::min<int64_t>((int64_t)span, se[d]);
+  int64_t cap = ext[0] * ext[1] * ext[2];
+  // zones larger than 2^21 nodes are computed inside the map kernel instead
+  cap = std::min<int64_t>(cap, (int64_t)1 << 21);
+  const size_t need = (size_t)cap * (size_t)n;
+  if (need > c->sfcode_elems) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const size_t alloc = std::max(need, c->sfcode_elems * 2);
+    dev_free(c->sfcode);
+    c->device_bytes -= (int64_t)(c->sfcode_elems * sizeof(unsigned short));
+    c->sfcode = nullptr;
+    c->sfcode_elems = 0;
+    rc = dev_alloc(c, &c->sfcode, alloc);
+    if (rc) return rc;
+    c->sfcode_elems = alloc;
+  }
+  c->zone_cap = (int)cap;
+  return LBMPSM_OK;
+}
+
 // Face table of `fix lbm-psm-bc`, src/fix_LBM_PSM_BC.cpp:223-401, in the
 // reference's halo'd coordinates, converted to stored coordinates.
 void add_face(lbmpsm_ctx* c, int kind, int axis, int plane_ref, int a_lo, int a_hi, int b_lo,
@@ -389,8 +419,7 @@ int run_map(lbmpsm_ctx* c) {
   pa.nbj = B.nbj;
   pa.nbk = B.nbk;
   {
-    ProfScope ps(c, "map_bin");
-    HIPCHK(c, launch_fill_int(B.count, B.nbricks, 0, c->stream));
+    ProfScope ps(c, "map_bin");  // counts are zero: set at create, reset by k_map
     HIPCHK(c, launch_prep_bin(pa, c->stream));
   }
   MapArgs ma{};
@@ -403,6 +432,7 @@ int run_map(lbmpsm_ctx* c) {
   ma.zone = P.zone;
   ma.list = B.list;
   ma.count = B.count;
+  ma.count_reset = B.count;
   ma.bi = B.bi;
   ma.bj = B.bj;
   ma.bk = B.bk;
@@ -595,6 +625,10 @@ int lbmpsm_create(const lbmpsm_config* cfg, lbmpsm_ctx** out) {
     lbmpsm_destroy(c);
     return fail(nullptr, LBMPSM_ERR_HIP, "hipEventCreate failed");
   }
+  if (launch_fill_int(B.count, B.nbricks, 0, c->stream) != hipSuccess) {
+    lbmpsm_destroy(c);
+    return fail(nullptr, LBMPSM_ERR_HIP, "brick count reset failed");
+  }
   *out = c;
   // reference start: rho = 1, u = 0 (src/fix_LBM_PSM.cpp:168)
   rc = lbmpsm_init_equilibrium(c, 1.0, 0.0, 0.0, 0.0);
@@ -709,35 +743,13 @@ int lbmpsm_set_particles(lbmpsm_ctx* c, int n, const int* tag, const double* x, 
   if (n == 0) return LBMPSM_OK;
   int rc = ensure_particles(c, n);
   if (rc) return rc;
-  // codes per entry: the largest zone box, (2 r/dx + 8) nodes per axis bounds
-  // every map zone (src/LBM_PSM_exchangeParticleData.cpp:41-48)
   double rmax = 0.0;
   for (int p = 0; p < n; ++p) {
     if (!(std::fabs(radius[p]) < 1e300)) return fail(c, LBMPSM_ERR_ARG, "radius is not finite");
     rmax = std::max(rmax, std::fabs(radius[p]));
   }
-  {
-    const double span = 2.0 * rmax / c->cfg.dx + 8.0;
-    int64_t ext[3];
-    const int se[3] = {c->g.sx, c->g.sy, c->g.sz};
-    for (int d = 0; d < 3; ++d) ext[d] = std::min<int64_t>((int64_t)span, se[d]);
-    int64_t cap = ext[0] * ext[1] * ext[2];
-    // zones larger than 2^21 nodes are computed inside the map kernel instead
-    cap = std::min<int64_t>(cap, (int64_t)1 << 21);
-    const size_t need = (size_t)cap * (size_t)n;
-    if (need > c->sfcode_elems) {
-      HIPCHK(c, hipStreamSynchronize(c->stream));
-      dev_free(c->sfcode);
-      c->device_bytes -= (int64_t)(c->sfcode_elems * sizeof(unsigned short));
-      c->sfcode = nullptr;
-      c->sfcode_elems = 0;
-      const size_t alloc = std::max(need, c->sfcode_elems * 2);
-      rc = dev_alloc(c, &c->sfcode, alloc);
-      if (rc) return rc;
-      c->sfcode_elems = alloc;
-    }
-    c->zone_cap = (int)cap;
-  }
+  rc = size_codes(c, n, rmax);
+  if (rc) return rc;
   // stage into pinned memory (after the previous upload has left it), one copy
   HIPCHK(c, hipEventSynchronize(c->stage_done));
   char* h = (char*)c->h_stage;
@@ -749,6 +761,26 @@ int lbmpsm_set_particles(lbmpsm_ctx* c, int n, const int* tag, const double* x, 
   std::memcpy(h + 80 * (size_t)n, tag, (size_t)n * sizeof(int));
   HIPCHK(c, hipMemcpyAsync(c->part.raw, h, 84 * (size_t)n, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipEventRecord(c->stage_done, c->stream));
+  return run_map(c);
+}
+
+int lbmpsm_set_particles_device(lbmpsm_ctx* c, int n, const int* tag, const double* x,
+                                const double* v, const double* omega, const double* radius,
+                                double max_radius) {
+  if (!c) return LBMPSM_ERR_ARG;
+  if (n < 0) return fail(c, LBMPSM_ERR_ARG, "negative particle count");
+  if (n > 0 && (!tag || !x || !v || !omega || !radius)) return fail(c, LBMPSM_ERR_ARG, "null particle array");
+  if (!(max_radius >= 0.0 && max_radius < 1e300))
+    return fail(c, LBMPSM_ERR_ARG, "max_radius must be finite and >= 0");
+  HIPCHK(c, hipSetDevice(c->device));
+  c->hydro_ready = 0;
+  c->part.n = n;
+  if (n == 0) return LBMPSM_OK;
+  int rc = ensure_particles(c, n);
+  if (rc) return rc;
+  rc = size_codes(c, n, max_radius);
+  if (rc) return rc;
+  HIPCHK(c, launch_pack_particles(n, tag, x, v, omega, radius, c->part.raw, c->stream));
   return run_map(c);
 }
 
@@ -902,7 +934,8 @@ int lbmpsm_sync(lbmpsm_ctx* c) {
 
 int lbmpsm_profile_enable(lbmpsm_ctx* c, int enable) {
   if (!c) return LBMPSM_ERR_ARG;
-  c->prof = enable ? 1 : 0;
+  if (enable < 0 || enable > 2) return fail(c, LBMPSM_ERR_ARG, "profile mode must be 0, 1 or 2");
+  c->prof = enable;
   return LBMPSM_OK;
 }
 

@@ -577,10 +577,24 @@ __device__ __forceinline__ void gather(const CollideArgs& a, int i, int j, int k
 // threads, so each boundary node is updated once, with all its faces
 // applied in table order.
 template <int DIM, int Q, int ST>
-__global__ __launch_bounds__(64) void k_collide_faces(const CollideArgs a, int fi) {
+// One launch covers every face: consecutive block ranges belong to faces in
+// application order.  A node shared by several faces is handled by the
+// first face that holds it; the faces only read the previous state, so
+// their order of execution does not matter.
+__global__ __launch_bounds__(64) void k_collide_faces(const CollideArgs a) {
+  int fi = 0;
+  int64_t b0 = 0;
+  for (; fi < a.nfaces; ++fi) {
+    const Face& G = a.faces[fi];
+    const int64_t cnt = (int64_t)max(G.a_hi - G.a_lo + 1, 0) * max(G.b_hi - G.b_lo + 1, 0);
+    const int64_t nblk = (cnt + 63) / 64;
+    if ((int64_t)blockIdx.x < b0 + nblk) break;
+    b0 += nblk;
+  }
+  if (fi >= a.nfaces) return;
   const Face F = a.faces[fi];
   const int na = F.a_hi - F.a_lo + 1, nb = F.b_hi - F.b_lo + 1;
-  const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  const int64_t t = ((int64_t)blockIdx.x - b0) * 64 + threadIdx.x;
   if (na <= 0 || nb <= 0 || t >= (int64_t)na * nb) return;
   const int ta = F.a_lo + (int)(t / nb), tb = F.b_lo + (int)(t % nb);
   int i, j, k;
@@ -712,18 +726,21 @@ static hipError_t collide_dim(int storage, const CollideArgs& a, hipStream_t s) 
 
 template <int DIM, int Q>
 static hipError_t collide_faces_dim(int storage, const CollideArgs& a, hipStream_t s) {
-  if (a.poststream) return hipSuccess;
+  if (a.poststream || a.nfaces <= 0) return hipSuccess;
+  int64_t blocks = 0;
   for (int fi = 0; fi < a.nfaces; ++fi) {
     const Face& F = a.host_faces[fi];
-    const int64_t cnt = (int64_t)(F.a_hi - F.a_lo + 1) * (F.b_hi - F.b_lo + 1);
-    if (cnt <= 0) continue;
-    dim3 fg((unsigned)((cnt + 63) / 64));
-    switch (storage) {
-      case ST_F64: hipLaunchKernelGGL((k_collide_faces<DIM, Q, ST_F64>), fg, dim3(64), 0, s, a, fi); break;
-      case ST_F32: hipLaunchKernelGGL((k_collide_faces<DIM, Q, ST_F32>), fg, dim3(64), 0, s, a, fi); break;
-      case ST_F16: hipLaunchKernelGGL((k_collide_faces<DIM, Q, ST_F16>), fg, dim3(64), 0, s, a, fi); break;
-      default: return hipErrorInvalidValue;
-    }
+    const int64_t cnt =
+        (int64_t)std::max(F.a_hi - F.a_lo + 1, 0) * std::max(F.b_hi - F.b_lo + 1, 0);
+    blocks += (cnt + 63) / 64;
+  }
+  if (blocks <= 0) return hipSuccess;
+  const dim3 fg((unsigned)blocks);
+  switch (storage) {
+    case ST_F64: hipLaunchKernelGGL((k_collide_faces<DIM, Q, ST_F64>), fg, dim3(64), 0, s, a); break;
+    case ST_F32: hipLaunchKernelGGL((k_collide_faces<DIM, Q, ST_F32>), fg, dim3(64), 0, s, a); break;
+    case ST_F16: hipLaunchKernelGGL((k_collide_faces<DIM, Q, ST_F16>), fg, dim3(64), 0, s, a); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

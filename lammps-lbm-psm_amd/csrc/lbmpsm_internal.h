@@ -109,6 +109,7 @@ struct MapArgs {
   const int* zone;
   const int* list;
   const int* count;
+  int* count_reset;        // the same counts, zeroed by k_map once read
   unsigned short* sfcode;  // [n][zone_cap] codes of each entry's zone, k fastest
   int zone_cap;            // codes per entry
   int bi, bj, bk, nbi, nbj, nbk;
@@ -151,6 +152,9 @@ struct PrepArgs {
   int bi, bj, bk, nbi, nbj, nbk;
 };
 hipError_t launch_fill_int(int* p, int64_t n, int value, hipStream_t s);
+// Device-resident particle arrays into ParticleDev::raw.
+hipError_t launch_pack_particles(int n, const int* tag, const double* x, const double* v,
+                                 const double* w, const double* r, char* raw, hipStream_t s);
 // Per-entry lattice quantities and brick lists (one wavefront per entry).
 hipError_t launch_prep_bin(const PrepArgs& a, hipStream_t s);
 // Solid fraction of every entry at every node of its map zone.

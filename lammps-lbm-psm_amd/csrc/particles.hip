@@ -217,16 +217,123 @@ __device__ __forceinline__ void map_transition(const MapArgs& a, const MapEntry&
 // chunk of kSfChunk consecutive zone nodes (grid = entries x chunks per
 // entry; chunks past the end of a zone exit):
 //  1. every thread classifies its nodes with the reference's early exits;
-//     cells that need the sub-point count go to a queue;
-//  2. groups of kMapGroup lanes count the queued cells, each lane taking a
-//     share of the 100 (a,b) sub-point pairs, and add the integer parts.
+//     cells that need the sub-point count are queued by column axis
+//     (sf_column_axis);
+//  2. the queued cells are counted in batches of kSfBatch cells, ten lanes
+//     per cell, one lane per slice of the lane axis: each lane settles the
+//     columns whose end points agree and queues the rest; the undecided
+//     columns are then counted one per lane (ten uniform tests each).
+// Every path sums the reference's own integer count (sf_count_columns).
 // Codes are stored k fastest over the zone box (index within the zone).
 constexpr int kSfChunk = 1024;
-constexpr int kMapGroup = 8;
+constexpr int kSfBatch = 24;
+constexpr int kSfUndecCap = kSfBatch * kSfPairs;
+static_assert(kSfBatch * kSfSlices <= 256, "one lane per cell slice");
+
+// Zones whose codes k_entry_sf stores (others are computed inline by k_map):
+// at most zone_cap nodes, and extents that fit the packed rim-cell queue.
+__device__ __forceinline__ bool zone_coded(int ni, int nj, int nk, int zone_cap) {
+  return ni > 0 && nj > 0 && nk > 0 && ni < 2048 && nj < 2048 && nk < 1024 &&
+         (int64_t)ni * nj * nk <= (int64_t)zone_cap;
+}
+
+struct SfShared {
+  int qnode[3][kSfChunk];       // rim cells per column axis: li | lj << 11 | lk << 22
+  int qlen[3];
+  double sq[kSfBatch][3][kSfSlices];  // lane, loop, column squares per cell
+  double cmm[kSfBatch][2];      // column min / max
+  double off[kSfBatch][3];      // dx, dy, dz of each cell
+  unsigned short upair[kSfUndecCap];  // (cell << 8) | (lane slice << 4) | loop slice
+  int ulen;
+  int cnt[kSfBatch];
+};
+
+__device__ __forceinline__ void sf_cell_offsets(const Geom& g, const int* z, int li, int lj,
+                                                int lk, double xL, double yL, double zL,
+                                                double& dx, double& dy, double& dz) {
+  dx = (double)(z[0] + li + g.off[0]) - xL;
+  dy = (double)(z[2] + lj + g.off[1]) - yL;
+  dz = (double)(z[4] + lk + g.off[2]) - zL;
+}
+
+template <int AXIS>
+__device__ __forceinline__ void sf_count_queue(const MapArgs& a, SfShared& sh, const int* z,
+                                               int nj, int nk, double xL, double yL, double zL,
+                                               double rSq, unsigned short* out) {
+  const int tid = threadIdx.x;
+  const int nq = sh.qlen[AXIS];
+  const bool two_d = a.g.dim == 2;
+  for (int b0 = 0; b0 < nq; b0 += kSfBatch) {
+    const int nb = min(kSfBatch, nq - b0);
+    // offsets of each cell (lane, loop, column order), then their squares
+    if (tid < nb) {
+      const int p = sh.qnode[AXIS][b0 + tid];
+      double dx, dy, dz, lane, loop, col;
+      sf_cell_offsets(a.g, z, p & 2047, (p >> 11) & 2047, p >> 22, xL, yL, zL, dx, dy, dz);
+      sf_axis_offsets(AXIS, dx, dy, dz, lane, loop, col);
+      sh.off[tid][0] = lane;
+      sh.off[tid][1] = loop;
+      sh.off[tid][2] = col;
+    }
+    if (tid < kSfBatch) sh.cnt[tid] = 0;
+    if (tid == 0) sh.ulen = 0;
+    __syncthreads();
+    for (int t = tid; t < nb * 3 * kSfSlices; t += 256) {
+      const int cell = t / (3 * kSfSlices);
+      const int rem = t - cell * 3 * kSfSlices;
+      const int arr = rem / kSfSlices, sl = rem - arr * kSfSlices;
+      const double o = sh.off[cell][arr] + sf_delta(sl);
+      sh.sq[cell][arr][sl] = o * o;
+    }
+    __syncthreads();
+    if (tid < nb) {
+      double lo = sh.sq[tid][2][0], hi = lo;
+      for (int c = 1; c < kSfSlices; ++c) {
+        const double v = sh.sq[tid][2][c];
+        lo = v < lo ? v : lo;
+        hi = v > hi ? v : hi;
+      }
+      sh.cmm[tid][0] = lo;
+      sh.cmm[tid][1] = hi;
+    }
+    __syncthreads();
+    if (tid < nb * kSfSlices) {
+      const int cell = tid / kSfSlices, l = tid - cell * kSfSlices;
+      int n;
+      if (two_d) {
+        n = sf_lane_count_2d(sh.sq[cell][0][l], sh.sq[cell][1], rSq);
+      } else {
+        unsigned undec = 0u;
+        n = sf_lane_count<AXIS>(sh.sq[cell][0][l], sh.sq[cell][1], sh.cmm[cell][0],
+                                sh.cmm[cell][1], rSq, undec);
+        if (undec) {
+          const int k0 = atomicAdd(&sh.ulen, __popc(undec));
+          int k = k0;
+          for (int m = 0; m < kSfSlices; ++m)
+            if (undec & (1u << m)) sh.upair[k++] = (unsigned short)((cell << 8) | (l << 4) | m);
+        }
+      }
+      atomicAdd(&sh.cnt[cell], n);
+    }
+    __syncthreads();
+    const int nu = sh.ulen;
+    for (int u = tid; u < nu; u += 256) {
+      const unsigned short p = sh.upair[u];
+      const int cell = p >> 8, l = (p >> 4) & 15, m = p & 15;
+      atomicAdd(&sh.cnt[cell],
+                sf_column_count<AXIS>(sh.sq[cell][0][l], sh.sq[cell][1][m], sh.sq[cell][2], rSq));
+    }
+    __syncthreads();
+    if (tid < nb) {
+      const int p = sh.qnode[AXIS][b0 + tid];
+      out[((p & 2047) * nj + ((p >> 11) & 2047)) * nk + (p >> 22)] = (unsigned short)sh.cnt[tid];
+    }
+    __syncthreads();
+  }
+}
 
 __global__ __launch_bounds__(256) void k_entry_sf(const MapArgs a) {
-  __shared__ int qnode[kSfChunk];
-  __shared__ int qlen;
+  __shared__ SfShared sh;
   const int chunks = (a.zone_cap + kSfChunk - 1) / kSfChunk;
   const int e = blockIdx.x / chunks;
   const int base = (blockIdx.x - e * chunks) * kSfChunk;
@@ -235,49 +342,40 @@ __global__ __launch_bounds__(256) void k_entry_sf(const MapArgs a) {
   const int* z = a.zone + 6 * e;
   const int ni = z[1] - z[0], nj = z[3] - z[2], nk = z[5] - z[4];
   if (ni <= 0 || nj <= 0 || nk <= 0) return;
+  if (!zone_coded(ni, nj, nk, a.zone_cap)) return;  // (computed inline by k_map)
   const int tot = ni * nj * nk, njk = nj * nk;
-  if (tot > a.zone_cap || base >= tot) return;  // (oversized zones: see k_map)
+  if (base >= tot) return;
   const double xL = a.xl[4 * e + 0], yL = a.xl[4 * e + 1], zL = a.xl[4 * e + 2];
   const double rL = a.xl[4 * e + 3];
   unsigned short* out = a.sfcode + (size_t)e * a.zone_cap;
   const int tid = threadIdx.x;
-  const int grp = tid / kMapGroup, gl = tid % kMapGroup;
-  {
-    if (tid == 0) qlen = 0;
-    __syncthreads();
-    const int cnt = min(kSfChunk, tot - base);
-    for (int t = tid; t < cnt; t += 256) {
-      const int li = (base + t) / njk;
-      const int rem = base + t - li * njk;
-      const int lj = rem / nk;
-      const int lk = rem - lj * nk;
-      const double dx = (double)(z[0] + li + g.off[0]) - xL;
-      const double dy = (double)(z[2] + lj + g.off[1]) - yL;
-      const double dz = (double)(z[4] + lk + g.off[2]) - zL;
-      const int cls = sf_classify(a.hsq_half, dx, dy, dz, rL);
-      if (cls == SF_ZERO) out[base + t] = 0;
-      else if (cls == SF_ONE) out[base + t] = kSfCodeOne;
-      else qnode[atomicAdd(&qlen, 1)] = base + t;
+  if (tid < 3) sh.qlen[tid] = 0;
+  __syncthreads();
+  const int cnt = min(kSfChunk, tot - base);
+  for (int t = tid; t < cnt; t += 256) {
+    const int idx = base + t;
+    const int li = idx / njk;
+    const int rem = idx - li * njk;
+    const int lj = rem / nk;
+    const int lk = rem - lj * nk;
+    double dx, dy, dz;
+    sf_cell_offsets(g, z, li, lj, lk, xL, yL, zL, dx, dy, dz);
+    const int cls = sf_classify(a.hsq_half, dx, dy, dz, rL);
+    if (cls == SF_ZERO) {
+      out[base + t] = 0;
+    } else if (cls == SF_ONE) {
+      out[base + t] = kSfCodeOne;
+    } else {
+      const int axis = g.dim == 2 ? 2 : sf_column_axis(dx, dy, dz);
+      sh.qnode[axis][atomicAdd(&sh.qlen[axis], 1)] = li | (lj << 11) | (lk << 22);
     }
-    __syncthreads();
-    const int nq = qlen;
-    for (int q0 = 0; q0 < nq; q0 += 256 / kMapGroup) {
-      const int q = q0 + grp;
-      int part = 0, idx = 0;
-      if (q < nq) {
-        idx = qnode[q];
-        const int li = idx / njk;
-        const int rem = idx - li * njk;
-        const int lj = rem / nk;
-        const int lk = rem - lj * nk;
-        const double dx = (double)(z[0] + li + g.off[0]) - xL;
-        const double dy = (double)(z[2] + lj + g.off[1]) - yL;
-        const double dz = (double)(z[4] + lk + g.off[2]) - zL;
-        part = sf_count_pairs(g.dim, dx, dy, dz, rL, gl, kSfPairs, kMapGroup);
-      }
-      for (int off = kMapGroup / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, kMapGroup);
-      if (q < nq && gl == 0) out[idx] = (unsigned short)part;
-    }
+  }
+  __syncthreads();
+  const double rSq = rL * rL;
+  sf_count_queue<2>(a, sh, z, nj, nk, xL, yL, zL, rSq, out);
+  if (g.dim == 3) {
+    sf_count_queue<0>(a, sh, z, nj, nk, xL, yL, zL, rSq, out);
+    sf_count_queue<1>(a, sh, z, nj, nk, xL, yL, zL, rSq, out);
   }
 }
 
@@ -351,9 +449,8 @@ __global__ __launch_bounds__(256) void k_map(const MapArgs a) {
         const int* z = E.zone;
         if (i < z[0] || i >= z[1] || j < z[2] || j >= z[3] || k < z[4] || k >= z[5]) continue;
         const int nj = z[3] - z[2], nk = z[5] - z[4];
-        const int tot = (z[1] - z[0]) * nj * nk;
         double sf;
-        if (tot <= a.zone_cap) {
+        if (zone_coded(z[1] - z[0], nj, nk, a.zone_cap)) {
           const int idx = ((i - z[0]) * nj + (j - z[2])) * nk + (k - z[4]);
           const unsigned short cd = a.sfcode[(size_t)entid[t] * a.zone_cap + idx];
           sf = cd == kSfCodeOne ? 1.0 : sf_from_count(g.dim, cd);
@@ -365,6 +462,8 @@ __global__ __launch_bounds__(256) void k_map(const MapArgs a) {
     }
     __syncthreads();
   }
+  // the brick list is consumed: leave its count at zero for the next map
+  if (tid == 0) a.count_reset[b] = 0;
   if (!valid) return;
   if (S.d0) {
     a.s.tag[n] = S.t0;
@@ -530,6 +629,32 @@ hipError_t launch_fill_int(int* p, int64_t n, int value, hipStream_t s) {
 hipError_t launch_prep_bin(const PrepArgs& a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_prep_bin, dim3(a.n), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+// Device-resident particle arrays into the packed per-call block
+// (x[n][3], v[n][3], omega[n][3], radius[n], tag[n]).
+__global__ __launch_bounds__(256) void k_pack_particles(int n, const int* tag, const double* x,
+                                                        const double* v, const double* w,
+                                                        const double* r, char* raw) {
+  const int t = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  double* d = (double*)raw;
+  if (t < 3 * n) {
+    d[t] = x[t];
+    d[3 * n + t] = v[t];
+    d[6 * n + t] = w[t];
+  }
+  if (t < n) {
+    d[9 * n + t] = r[t];
+    ((int*)(raw + 80 * (size_t)n))[t] = tag[t];
+  }
+}
+
+hipError_t launch_pack_particles(int n, const int* tag, const double* x, const double* v,
+                                 const double* w, const double* r, char* raw, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pack_particles, dim3((unsigned)((3 * n + 255) / 256)), dim3(256), 0, s, n,
+                     tag, x, v, w, r, raw);
   return hipGetLastError();
 }
 

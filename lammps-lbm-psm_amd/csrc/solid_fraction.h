@@ -84,6 +84,129 @@ LBM_HD int sf_count_pairs(int dim, double dx, double dy, double dz, double rP_LB
   return n;
 }
 
+// ---------------------------------------------------------------------------
+// Count by columns along a chosen axis (the map kernel's form).
+//
+// Every sub-point test is the reference's fl(fl(X + Y) + Z) < r^2 with
+// X, Y, Z the squared sub-point offsets along x, y, z.  That value is
+// monotone in each of X, Y, Z, so along any axis a column of 10 points is
+// all-in when its largest square passes and all-out when its smallest fails;
+// only the columns in between are tested point by point.  The column axis
+// is free: choosing the one with the smallest |offset| (the axis most nearly
+// tangent to the sphere at the cell) leaves the fewest undecided columns.
+// The other two axes are the "lane" axis (one per lane in the kernel) and
+// the "loop" axis:
+//   column z: lane x, loop y     column x: lane y, loop z     column y: lane x, loop z
+// ---------------------------------------------------------------------------
+
+// Squared offsets of the 10 sub-point slices along one axis.
+LBM_HD void sf_squares(double d, double* sq) {
+  for (int c = 0; c < kSfSlices; ++c) {
+    const double t = d + sf_delta(c);
+    sq[c] = t * t;
+  }
+}
+
+LBM_HD int sf_column_axis(double dx, double dy, double dz) {
+  const double ax = dx < 0.0 ? -dx : dx, ay = dy < 0.0 ? -dy : dy, az = dz < 0.0 ? -dz : dz;
+  if (az <= ax && az <= ay) return 2;
+  if (ax <= ay) return 0;
+  return 1;
+}
+
+// Offsets (dx, dy, dz) of the lane, loop and column axes for a column axis.
+LBM_HD void sf_axis_offsets(int axis, double dx, double dy, double dz, double& lane,
+                            double& loop, double& col) {
+  if (axis == 2) {
+    lane = dx; loop = dy; col = dz;
+  } else if (axis == 0) {
+    lane = dy; loop = dz; col = dx;
+  } else {
+    lane = dx; loop = dz; col = dy;
+  }
+}
+
+// fl(fl(X + Y) + Z) from the lane, loop and column squares.
+template <int AXIS>
+LBM_HD double sf_point(double lane_sq, double loop_sq, double col_sq) {
+  if constexpr (AXIS == 2) return (lane_sq + loop_sq) + col_sq;
+  else if constexpr (AXIS == 0) return (col_sq + lane_sq) + loop_sq;
+  else return (lane_sq + col_sq) + loop_sq;
+}
+
+// Decided columns of one lane slice: returns their count and marks the
+// undecided loop indices in `undec`.
+template <int AXIS>
+LBM_HD int sf_lane_count(double lane_sq, const double* loop_sq, double col_min, double col_max,
+                         double rSq, unsigned& undec) {
+  int n = 0;
+  undec = 0u;
+  for (int m = 0; m < kSfSlices; ++m) {
+    if (!(sf_point<AXIS>(lane_sq, loop_sq[m], col_min) < rSq)) continue;
+    if (sf_point<AXIS>(lane_sq, loop_sq[m], col_max) < rSq) {
+      n += kSfSlices;
+      continue;
+    }
+    undec |= 1u << m;
+  }
+  return n;
+}
+
+template <int AXIS>
+LBM_HD int sf_column_count(double lane_sq, double loop_sq, const double* col_sq, double rSq) {
+  int n = 0;
+  for (int c = 0; c < kSfSlices; ++c) n += sf_point<AXIS>(lane_sq, loop_sq, col_sq[c]) < rSq;
+  return n;
+}
+
+// 2D: all 100 tests fl(X + Y) < r^2 of one lane slice.
+LBM_HD int sf_lane_count_2d(double lane_sq, const double* loop_sq, double rSq) {
+  int n = 0;
+  for (int m = 0; m < kSfSlices; ++m) n += (lane_sq + loop_sq[m]) < rSq;
+  return n;
+}
+
+// Whole-cell count in the column form, with the column axis given
+// (axis < 0: the kernel's choice).  Host reference of the kernel's split.
+template <int AXIS>
+LBM_HD int sf_count_columns_axis(double lane, double loop, double col, double rSq) {
+  double ls[kSfSlices], ms[kSfSlices], cs[kSfSlices];
+  sf_squares(lane, ls);
+  sf_squares(loop, ms);
+  sf_squares(col, cs);
+  double cmin = cs[0], cmax = cs[0];
+  for (int c = 1; c < kSfSlices; ++c) {
+    cmin = cs[c] < cmin ? cs[c] : cmin;
+    cmax = cs[c] > cmax ? cs[c] : cmax;
+  }
+  int n = 0;
+  for (int l = 0; l < kSfSlices; ++l) {
+    unsigned undec = 0u;
+    n += sf_lane_count<AXIS>(ls[l], ms, cmin, cmax, rSq, undec);
+    for (int m = 0; m < kSfSlices; ++m)
+      if (undec & (1u << m)) n += sf_column_count<AXIS>(ls[l], ms[m], cs, rSq);
+  }
+  return n;
+}
+
+LBM_HD int sf_count_columns(int dim, double dx, double dy, double dz, double rP_LB, int axis) {
+  const double rSq = rP_LB * rP_LB;
+  if (dim == 2) {
+    double xs[kSfSlices], ys[kSfSlices];
+    sf_squares(dx, xs);
+    sf_squares(dy, ys);
+    int n = 0;
+    for (int l = 0; l < kSfSlices; ++l) n += sf_lane_count_2d(xs[l], ys, rSq);
+    return n;
+  }
+  if (axis < 0) axis = sf_column_axis(dx, dy, dz);
+  double lane, loop, col;
+  sf_axis_offsets(axis, dx, dy, dz, lane, loop, col);
+  if (axis == 2) return sf_count_columns_axis<2>(lane, loop, col, rSq);
+  if (axis == 0) return sf_count_columns_axis<0>(lane, loop, col, rSq);
+  return sf_count_columns_axis<1>(lane, loop, col, rSq);
+}
+
 // fraction * count, :120-124 and :168.
 LBM_HD double sf_from_count(int dim, int n) {
   double fraction = 1.0 / ((double)(kSfSlices * kSfSlices));

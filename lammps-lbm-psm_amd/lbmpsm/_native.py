@@ -104,6 +104,8 @@ def _load() -> C.CDLL:
         "lbmpsm_init_equilibrium": (C.c_int, [P, C.c_double, C.c_double, C.c_double, C.c_double]),
         "lbmpsm_set_bc": (C.c_int, [P, C.c_int, C.c_double, C.c_double]),
         "lbmpsm_set_particles": (C.c_int, [P, C.c_int, ip, dp, dp, dp, dp]),
+        "lbmpsm_set_particles_device": (C.c_int, [P, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                  C.c_void_p, C.c_void_p, C.c_double]),
         "lbmpsm_step": (C.c_int, [P, C.c_int]),
         "lbmpsm_get_hydro": (C.c_int, [P, C.c_int, dp, dp, dp]),
         "lbmpsm_get_populations": (C.c_int, [P, dp]),

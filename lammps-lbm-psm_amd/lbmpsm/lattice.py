@@ -119,6 +119,13 @@ class Lattice:
                                                N.dptr(w), N.dptr(r)))
         self._n = n
 
+    def set_particles_device(self, n, tag_ptr, x_ptr, v_ptr, omega_ptr, radius_ptr, max_radius):
+        """set_particles for a list already in device memory (raw device
+        addresses of int32 tag[n] and float64 x/v/omega[n][3], radius[n])."""
+        self._check(N.LIB.lbmpsm_set_particles_device(self._h, int(n), tag_ptr, x_ptr, v_ptr,
+                                                      omega_ptr, radius_ptr, float(max_radius)))
+        self._n = int(n)
+
     def step(self, store_macros=False):
         self._check(N.LIB.lbmpsm_step(self._h, N.STEP_STORE_MACROS if store_macros else 0))
 
@@ -207,7 +214,9 @@ class Lattice:
 
     # -- timing ---------------------------------------------------------------
     def profile(self, enable=True):
-        self._check(N.LIB.lbmpsm_profile_enable(self._h, 1 if enable else 0))
+        """True: time every kernel; "collide": the bulk collide only; False: off."""
+        mode = 2 if enable == "collide" else (1 if enable else 0)
+        self._check(N.LIB.lbmpsm_profile_enable(self._h, mode))
 
     def profile_reset(self):
         self._check(N.LIB.lbmpsm_profile_reset(self._h))

@@ -96,6 +96,8 @@ def sfcheck():
     lib.sf_fast.argtypes = args
     lib.sf_split.restype = C.c_double
     lib.sf_split.argtypes = args + [C.c_int]
+    lib.sf_columns.restype = C.c_double
+    lib.sf_columns.argtypes = args + [C.c_int]
     return lib
 
 
@@ -122,6 +124,8 @@ def test_library_solid_fraction_equals_oracle(sfcheck, dim):
         assert sfcheck.sf_fast(dim, hs, *args) == expect
         for groups in (1, 8, 16, 64):
             assert sfcheck.sf_split(dim, hs, *args, groups) == expect
+        for axis in (-1, 0, 1, 2):
+            assert sfcheck.sf_columns(dim, hs, *args, axis) == expect
 
 
 def test_ghost_images_corner():
