@@ -54,6 +54,7 @@ struct lbmpsm_ctx {
   BrickDev br;
   unsigned short* sfcode = nullptr;  // per-entry zone solid-fraction codes
   int zone_cap = 0;                  // codes per entry
+  int zone_span = 0;                 // zone extent bound per axis (nodes)
   size_t sfcode_elems = 0;
   // pinned host staging for one particle upload and the hydro download
   void* h_stage = nullptr;
@@ -325,6 +326,7 @@ int size_codes(lbmpsm_ctx* c, int n, double rmax) {
     c->sfcode_elems = alloc;
   }
   c->zone_cap = (int)cap;
+  c->zone_span = (int)std::min<double>(span, 2147483647.0);
   return LBMPSM_OK;
 }
 
@@ -442,6 +444,7 @@ int run_map(lbmpsm_ctx* c) {
   ma.hsq_half = c->g.dim == 3 ? std::sqrt(3.1) / 2.0 : std::sqrt(2.1) / 2.0;
   ma.sfcode = c->sfcode;
   ma.zone_cap = c->zone_cap;
+  ma.zone_span = c->zone_span;
   {
     ProfScope ps(c, "map_fraction");
     HIPCHK(c, launch_entry_sf(ma, c->stream));

@@ -112,6 +112,7 @@ struct MapArgs {
   int* count_reset;        // the same counts, zeroed by k_map once read
   unsigned short* sfcode;  // [n][zone_cap] codes of each entry's zone, k fastest
   int zone_cap;            // codes per entry
+  int zone_span;           // bound of every zone's extent per axis
   int bi, bj, bk, nbi, nbj, nbk;
   double hsq_half;
 };

@@ -16,6 +16,8 @@
 //    with a fixed-order tree reduction.
 //
 // Compiled with -ffp-contract=off: all geometry is the reference's FP64.
+#include <algorithm>
+
 #include "lbmpsm_internal.h"
 #include "solid_fraction.h"
 
@@ -226,15 +228,19 @@ __device__ __forceinline__ void map_transition(const MapArgs& a, const MapEntry&
 // Every path sums the reference's own integer count (sf_count_columns).
 // Codes are stored k fastest over the zone box (index within the zone).
 constexpr int kSfChunk = 1024;
+constexpr int kLineMax = 32;  // y/z zone extent of the line kernel (k_entry_sf_lines)
 constexpr int kSfBatch = 24;
 constexpr int kSfUndecCap = kSfBatch * kSfPairs;
 static_assert(kSfBatch * kSfSlices <= 256, "one lane per cell slice");
 
 // Zones whose codes k_entry_sf stores (others are computed inline by k_map):
-// at most zone_cap nodes, and extents that fit the packed rim-cell queue.
-__device__ __forceinline__ bool zone_coded(int ni, int nj, int nk, int zone_cap) {
+// at most zone_cap nodes, at most zone_span per axis (the bound the code
+// tables and launch grids are sized for), and extents that fit the packed
+// rim-cell queue.
+__device__ __forceinline__ bool zone_coded(const MapArgs& a, int ni, int nj, int nk) {
   return ni > 0 && nj > 0 && nk > 0 && ni < 2048 && nj < 2048 && nk < 1024 &&
-         (int64_t)ni * nj * nk <= (int64_t)zone_cap;
+         ni <= a.zone_span && nj <= a.zone_span && nk <= a.zone_span &&
+         (int64_t)ni * nj * nk <= (int64_t)a.zone_cap;
 }
 
 struct SfShared {
@@ -342,7 +348,8 @@ __global__ __launch_bounds__(256) void k_entry_sf(const MapArgs a) {
   const int* z = a.zone + 6 * e;
   const int ni = z[1] - z[0], nj = z[3] - z[2], nk = z[5] - z[4];
   if (ni <= 0 || nj <= 0 || nk <= 0) return;
-  if (!zone_coded(ni, nj, nk, a.zone_cap)) return;  // (computed inline by k_map)
+  if (!zone_coded(a, ni, nj, nk)) return;  // (computed inline by k_map)
+  if (g.dim == 3 && nj <= kLineMax && nk <= kLineMax) return;  // (k_entry_sf_lines)
   const int tot = ni * nj * nk, njk = nj * nk;
   if (base >= tot) return;
   const double xL = a.xl[4 * e + 0], yL = a.xl[4 * e + 1], zL = a.xl[4 * e + 2];
@@ -379,25 +386,160 @@ __global__ __launch_bounds__(256) void k_entry_sf(const MapArgs a) {
   }
 }
 
-// One workgroup (256 threads = one brick of 256 nodes) per brick.  The
-// entries to replay are taken in list order: the brick's own list, sorted,
-// or - when more entries touch the brick than its list holds - the whole
-// particle list, staged kMapChunk at a time.  Every node replays them with
-// the solid fractions from k_entry_sf.
-constexpr int kMapChunk = 32;
+// 3D solid fractions by lines (sf_first_pass / sf_last_pass): one workgroup
+// per (entry, x plane of its zone).  The plane's y and z sub-point squares
+// and the interval of every (a, j, b) line go through LDS; each cell of the
+// plane then gets its code from the reference's early exits or, when it
+// needs counting, from the summed overlaps.  Zones wider than kLineMax in y
+// or z are left to k_entry_sf.
+struct SfLineShared {
+  double X[kSfSlices];
+  double Y[kLineMax * kSfSlices];
+  double Z[kLineMax * kSfSlices];
+  int part[kLineMax][kLineMax + 1];
+  int diff[kLineMax][kLineMax + 1];
+  int pstar;
+  double t0;   // z offset of the line's first point
+};
 
-__global__ __launch_bounds__(256) void k_map(const MapArgs a) {
-  __shared__ int order[kBrickCap];
+__global__ __launch_bounds__(256) void k_entry_sf_lines(const MapArgs a, int planes) {
+  __shared__ SfLineShared sh;
+  const int e = (int)blockIdx.x / planes;
+  const int li = (int)blockIdx.x - e * planes;
+  if (e >= a.n) return;
+  const Geom& g = a.g;
+  const int* z = a.zone + 6 * e;
+  const int ni = z[1] - z[0], nj = z[3] - z[2], nk = z[5] - z[4];
+  if (!zone_coded(a, ni, nj, nk) || li >= ni) return;
+  const double xL = a.xl[4 * e + 0], yL = a.xl[4 * e + 1], zL = a.xl[4 * e + 2];
+  const double rL = a.xl[4 * e + 3];
+  const double rSq = rL * rL;
+  unsigned short* out = a.sfcode + (size_t)e * a.zone_cap + (size_t)li * nj * nk;
+  const int tid = threadIdx.x;
+  const double dx = (double)(z[0] + li + g.off[0]) - xL;
+
+  if (nj > kLineMax || nk > kLineMax) return;  // (k_entry_sf counts these)
+
+  const int P = kSfSlices * nk;
+  for (int t = tid; t < kSfSlices * (1 + nj + nk); t += 256) {
+    const int sl = t % kSfSlices, row = t / kSfSlices;
+    double d;
+    double* dst;
+    if (row == 0) {
+      d = dx;
+      dst = sh.X + sl;
+    } else if (row <= nj) {
+      d = (double)(z[2] + (row - 1) + g.off[1]) - yL;
+      dst = sh.Y + kSfSlices * (row - 1) + sl;
+    } else {
+      d = (double)(z[4] + (row - 1 - nj) + g.off[2]) - zL;
+      dst = sh.Z + kSfSlices * (row - 1 - nj) + sl;
+    }
+    const double o = d + sf_delta(sl);
+    *dst = o * o;
+    if (row == 1 + nj && sl == 0) sh.t0 = o;
+  }
+  for (int t = tid; t < nj * (kLineMax + 1); t += 256) {
+    (&sh.part[0][0])[t] = 0;
+    (&sh.diff[0][0])[t] = 0;
+  }
+  __syncthreads();
+  // the line's minimum: along the V, the first point of the lowest value is
+  // the only one below its predecessor and not above its successor
+  for (int p = tid; p < P; p += 256) {
+    const double zp = sh.Z[p];
+    if ((p == 0 || sh.Z[p - 1] > zp) && (p == P - 1 || sh.Z[p + 1] >= zp)) sh.pstar = p;
+  }
+  __syncthreads();
+  const int pstar = sh.pstar;
+  const double t0 = sh.t0;
+  // column fastest across lanes: lanes of a wave add into different columns
+  int ab = tid / nj, j = tid - ab * nj;
+  const int dab = 256 / nj, dj = 256 - dab * nj;
+  for (; ab < kSfPairs; ab += dab, j += dj) {
+    if (j >= nj) {
+      j -= nj;
+      ++ab;
+      if (ab >= kSfPairs) break;
+    }
+    const int ia = ab / kSfSlices, ib = ab - ia * kSfSlices;
+    const double sxy = sh.X[ia] + sh.Y[kSfSlices * j + ib];
+    int lo, hi;
+    if (!sf_line_interval(sh.Z, P, pstar, t0, sxy, rSq, lo, hi)) continue;
+    int klo, khi, nlo, nhi;
+    sf_line_split(lo, hi, klo, khi, nlo, nhi);
+    atomicAdd(&sh.part[j][klo], nlo);
+    if (khi > klo) {
+      atomicAdd(&sh.part[j][khi], nhi);
+      if (khi > klo + 1) {
+        atomicAdd(&sh.diff[j][klo + 1], kSfSlices);
+        atomicAdd(&sh.diff[j][khi], -kSfSlices);
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < nj) {
+    int run = 0;
+    for (int k = 0; k < nk; ++k) {
+      run += sh.diff[tid][k];
+      sh.part[tid][k] += run;
+    }
+  }
+  __syncthreads();
+  for (int t = tid; t < nj * nk; t += 256) {
+    const int j = t / nk, k = t - (t / nk) * nk;
+    const double dy = (double)(z[2] + j + g.off[1]) - yL;
+    const double dz = (double)(z[4] + k + g.off[2]) - zL;
+    const int cls = sf_classify(a.hsq_half, dx, dy, dz, rL);
+    out[t] = cls == SF_ZERO ? 0 : cls == SF_ONE ? kSfCodeOne : (unsigned short)sh.part[j][k];
+  }
+}
+
+// One workgroup (256 threads = one brick of 256 nodes) per brick.  The
+// entries to replay are taken in list order: the brick's own list, staged
+// whole and ranked, or - when more entries touch the brick than its list
+// holds - the whole particle list, staged kBrickCap at a time.  Every node
+// replays them with the solid fractions from k_entry_sf.  The loads that do
+// not depend on each other are issued together: count, list and the node's
+// slots first, then all listed entries, then the codes of a batch of
+// entries.
+__device__ __forceinline__ double map_code_sf(int dim, unsigned short cd) {
+  return cd == kSfCodeOne ? 1.0 : sf_from_count(dim, cd);
+}
+
+// Entries whose solid-fraction codes a node loads together before applying
+// them in order (their loads are independent, so they overlap), and whether
+// the solid fractions of occupied slots are loaded up front (1) or when an
+// entry of the same tag first needs them (0).
+#ifndef LBMPSM_MAP_BATCH
+#define LBMPSM_MAP_BATCH 2
+#endif
+#ifndef LBMPSM_MAP_EAGER
+#define LBMPSM_MAP_EAGER 1
+#endif
+constexpr int kMapBatch = LBMPSM_MAP_BATCH;
+#ifndef LBMPSM_MAP_WAVES
+#define LBMPSM_MAP_WAVES 0  // > 0: ask for this many waves per SIMD for k_map
+#endif
+#if LBMPSM_MAP_WAVES > 0
+#define LBMPSM_MAP_OCC __attribute__((amdgpu_waves_per_eu(LBMPSM_MAP_WAVES)))
+#else
+#define LBMPSM_MAP_OCC
+#endif
+
+__global__ __launch_bounds__(256) LBMPSM_MAP_OCC void k_map(const MapArgs a) {
+  __shared__ MapEntry ent[kBrickCap];
   __shared__ int raw[kBrickCap];
-  __shared__ MapEntry ent[kMapChunk];
-  __shared__ int entid[kMapChunk];
+  __shared__ int perm[kBrickCap];   // list order -> staging slot
   const int64_t b = blockIdx.x;
+  const int tid = threadIdx.x;
+  // independent loads first: the brick's count and list, the node's slots
   const int m = a.count[b];
+  const int rv = tid < kBrickCap ? a.list[b * kBrickCap + tid] : 0;
   if (m == 0) return;
   const Geom& g = a.g;
   const int64_t plane = g.plane;
   const bool listed = m <= kBrickCap;
-  const int tid = threadIdx.x;
 
   const int bb = (int)b;
   const int bx = bb / (a.nbj * a.nbk);
@@ -412,55 +554,106 @@ __global__ __launch_bounds__(256) void k_map(const MapArgs a) {
   NodeSlots S;
   S.t0 = valid ? a.s.tag[n] : 0;
   S.t1 = valid ? a.s.tag[plane + n] : 0;
-  S.sf0 = S.sf1 = 0.0;
-  S.d0 = S.d1 = S.k0 = S.k1 = false;
+  S.d0 = S.d1 = false;
   for (int c = 0; c < 3; ++c) S.u0[c] = S.u1[c] = 0.0;
+#if LBMPSM_MAP_EAGER
+  // solid fractions of occupied slots are needed whenever an entry of the
+  // same tag arrives; load them now, alongside the list
+  S.sf0 = S.t0 != 0 ? a.s.sf[n] : 0.0;
+  S.sf1 = S.t1 != 0 ? a.s.sf[plane + n] : 0.0;
+  S.k0 = S.k1 = true;
+#else
+  S.sf0 = S.sf1 = 0.0;
+  S.k0 = S.k1 = false;
+#endif
   // reference (halo'd) node indices, as used in the geometry
   const int ri = i + g.off[0], rj = j + g.off[1], rk = k + g.off[2];
 
   if (listed) {
-    // entries are distinct, so rank = number of smaller entries
-    if (tid < m) raw[tid] = a.list[b * kBrickCap + tid];
-    __syncthreads();
+    // stage every listed entry at once (list slot = staging slot) and rank
+    // the list, entries being distinct: rank = number of smaller entries
     if (tid < m) {
-      const int v = raw[tid];
-      int r = 0;
-      for (int u = 0; u < m; ++u) r += raw[u] < v;
-      order[r] = v;
+      raw[tid] = rv;
+      MapEntry& E = ent[tid];
+      for (int c = 0; c < 6; ++c) E.zone[c] = a.zone[6 * rv + c];
+      E.tag = a.tag[rv];
+      for (int c = 0; c < 4; ++c) E.xl[c] = a.xl[4 * rv + c];
+      for (int c = 0; c < 6; ++c) E.kin[c] = a.kin[6 * rv + c];
     }
     __syncthreads();
-  }
-  const int total = listed ? m : a.n;
-  for (int base = 0; base < total; base += kMapChunk) {
-    const int cnt = min(kMapChunk, total - base);
-    if (tid < cnt) {
-      const int v = listed ? order[base + tid] : base + tid;
-      MapEntry& E = ent[tid];
-      for (int c = 0; c < 6; ++c) E.zone[c] = a.zone[6 * v + c];
-      E.tag = a.tag[v];
-      for (int c = 0; c < 4; ++c) E.xl[c] = a.xl[4 * v + c];
-      for (int c = 0; c < 6; ++c) E.kin[c] = a.kin[6 * v + c];
-      entid[tid] = v;
+    if (tid < m) {
+      int r = 0;
+      for (int u = 0; u < m; ++u) r += raw[u] < rv;
+      perm[r] = tid;
     }
     __syncthreads();
     if (valid) {
-      for (int t = 0; t < cnt; ++t) {
-        const MapEntry& E = ent[t];
-        const int* z = E.zone;
-        if (i < z[0] || i >= z[1] || j < z[2] || j >= z[3] || k < z[4] || k >= z[5]) continue;
-        const int nj = z[3] - z[2], nk = z[5] - z[4];
-        double sf;
-        if (zone_coded(z[1] - z[0], nj, nk, a.zone_cap)) {
-          const int idx = ((i - z[0]) * nj + (j - z[2])) * nk + (k - z[4]);
-          const unsigned short cd = a.sfcode[(size_t)entid[t] * a.zone_cap + idx];
-          sf = cd == kSfCodeOne ? 1.0 : sf_from_count(g.dim, cd);
-        } else {
-          sf = solid_fraction(g.dim, a.hsq_half, ri, rj, rk, E.xl[0], E.xl[1], E.xl[2], E.xl[3]);
+      for (int r0 = 0; r0 < m; r0 += kMapBatch) {
+        unsigned short cd[kMapBatch];
+        unsigned in = 0u, coded = 0u;
+#pragma unroll
+        for (int u = 0; u < kMapBatch; ++u) {
+          cd[u] = 0;
+          if (r0 + u < m) {
+            const int sidx = perm[r0 + u];
+            const int* z = ent[sidx].zone;
+            if (i >= z[0] && i < z[1] && j >= z[2] && j < z[3] && k >= z[4] && k < z[5]) {
+              in |= 1u << u;
+              const int nj = z[3] - z[2], nk = z[5] - z[4];
+              if (zone_coded(a, z[1] - z[0], nj, nk)) {
+                coded |= 1u << u;
+                const int idx = ((i - z[0]) * nj + (j - z[2])) * nk + (k - z[4]);
+                cd[u] = a.sfcode[(size_t)raw[sidx] * a.zone_cap + idx];
+              }
+            }
+          }
         }
-        map_transition(a, E, sf, ri, rj, rk, n, S);
+#pragma unroll
+        for (int u = 0; u < kMapBatch; ++u) {
+          if (!(in & (1u << u))) continue;
+          const MapEntry& E = ent[perm[r0 + u]];
+          const double sf = (coded & (1u << u))
+                                ? map_code_sf(g.dim, cd[u])
+                                : solid_fraction(g.dim, a.hsq_half, ri, rj, rk, E.xl[0], E.xl[1],
+                                                 E.xl[2], E.xl[3]);
+          map_transition(a, E, sf, ri, rj, rk, n, S);
+        }
       }
     }
     __syncthreads();
+  } else {
+    // more entries touch the brick than its list holds: replay the whole
+    // particle list, staged kBrickCap entries at a time
+    for (int base = 0; base < a.n; base += kBrickCap) {
+      const int cnt = min(kBrickCap, a.n - base);
+      if (tid < cnt) {
+        const int v = base + tid;
+        MapEntry& E = ent[tid];
+        for (int c = 0; c < 6; ++c) E.zone[c] = a.zone[6 * v + c];
+        E.tag = a.tag[v];
+        for (int c = 0; c < 4; ++c) E.xl[c] = a.xl[4 * v + c];
+        for (int c = 0; c < 6; ++c) E.kin[c] = a.kin[6 * v + c];
+      }
+      __syncthreads();
+      if (valid) {
+        for (int t = 0; t < cnt; ++t) {
+          const MapEntry& E = ent[t];
+          const int* z = E.zone;
+          if (i < z[0] || i >= z[1] || j < z[2] || j >= z[3] || k < z[4] || k >= z[5]) continue;
+          const int nj = z[3] - z[2], nk = z[5] - z[4];
+          double sf;
+          if (zone_coded(a, z[1] - z[0], nj, nk)) {
+            const int idx = ((i - z[0]) * nj + (j - z[2])) * nk + (k - z[4]);
+            sf = map_code_sf(g.dim, a.sfcode[(size_t)(base + t) * a.zone_cap + idx]);
+          } else {
+            sf = solid_fraction(g.dim, a.hsq_half, ri, rj, rk, E.xl[0], E.xl[1], E.xl[2],
+                                E.xl[3]);
+          }
+          map_transition(a, E, sf, ri, rj, rk, n, S);
+        }
+      }
+      __syncthreads();
+    }
   }
   // the brick list is consumed: leave its count at zero for the next map
   if (tid == 0) a.count_reset[b] = 0;
@@ -488,8 +681,16 @@ __global__ __launch_bounds__(256) void k_map(const MapArgs a) {
   if (S.d0 || S.d1) a.s.mask[n] = (uint8_t)((S.t0 > 0 ? 1 : 0) | (S.t1 > 0 ? 2 : 0));
 }
 
-// Force, torque and stresslet of one list entry; one workgroup per entry.
-__global__ __launch_bounds__(256) void k_hydro(const HydroArgs a) {
+// Force, torque and stresslet of one list entry per workgroup of 1024
+// threads (16 waves): each thread sums its nodes of the entry's reduction
+// zone, then the sums are combined in a fixed order - a shuffle tree within
+// each wave, then the 16 wave sums in wave order.  Terms as
+// src/LBM_PSM_exchangeParticleData.cpp:197-240: F += Fhyd ff,
+// T += (r x Fhyd) tf, S -= 1/2 (r_a F_b + r_b F_a) tf with r the node
+// offset from the entry's lattice position.
+constexpr int kHydroThreads = 1024;
+
+__global__ __launch_bounds__(kHydroThreads) void k_hydro(const HydroArgs a) {
   const int p = blockIdx.x;
   if (p >= a.n) return;
   const Geom& g = a.g;
@@ -505,12 +706,12 @@ __global__ __launch_bounds__(256) void k_hydro(const HydroArgs a) {
   const int njk = nj * nk;
   // U nodes per thread per pass so that their loads are in flight together;
   // 32-bit index arithmetic (a zone holds far fewer than 2^31 nodes)
-  constexpr int U = 8;
-  for (int base = threadIdx.x; base < tot; base += 256 * U) {
+  constexpr int U = 4;
+  for (int base = threadIdx.x; base < tot; base += kHydroThreads * U) {
     int nn[U], sl[U], ii[U], jj[U], kk[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int t = base + 256 * u;
+      const int t = base + kHydroThreads * u;
       sl[u] = -1;
       nn[u] = 0;
       ii[u] = jj[u] = kk[u] = 0;
@@ -551,15 +752,19 @@ __global__ __launch_bounds__(256) void k_hydro(const HydroArgs a) {
       acc[11] -= 0.5 * (dz * F1 + dy * F2) * tf;
     }
   }
-  __shared__ double red[12][256];
-  for (int c = 0; c < 12; ++c) red[c][threadIdx.x] = acc[c];
+#pragma unroll
+  for (int c = 0; c < 12; ++c)
+    for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+  __shared__ double wsum[kHydroThreads / 64][12];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0)
+    for (int c = 0; c < 12; ++c) wsum[wave][c] = acc[c];
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
-      for (int c = 0; c < 12; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
-    __syncthreads();
+  if (threadIdx.x < 12) {
+    double s = 0.0;
+    for (int w = 0; w < kHydroThreads / 64; ++w) s += wsum[w][threadIdx.x];
+    a.out[12 * p + threadIdx.x] = s;
   }
-  if (threadIdx.x < 12) a.out[12 * p + threadIdx.x] = red[threadIdx.x][0];
 }
 
 __device__ __forceinline__ bool interior_of(const Geom& g, int64_t n, int64_t& o) {
@@ -660,8 +865,16 @@ hipError_t launch_pack_particles(int n, const int* tag, const double* x, const d
 
 hipError_t launch_entry_sf(const MapArgs& a, hipStream_t s) {
   if (a.n <= 0 || a.zone_cap <= 0) return hipSuccess;
-  const int64_t chunks = (a.zone_cap + kSfChunk - 1) / kSfChunk;
-  hipLaunchKernelGGL(k_entry_sf, dim3((unsigned)(a.n * chunks)), dim3(256), 0, s, a);
+  if (a.g.dim == 3) {
+    // every coded zone is at most zone_span planes wide in x
+    const int planes = std::max(1, std::min(a.zone_span, a.g.sx));
+    hipLaunchKernelGGL(k_entry_sf_lines, dim3((unsigned)((int64_t)a.n * planes)), dim3(256), 0,
+                       s, a, planes);
+  }
+  if (a.g.dim == 2 || a.zone_span > kLineMax) {  // zones the line kernel leaves
+    const int64_t chunks = (a.zone_cap + kSfChunk - 1) / kSfChunk;
+    hipLaunchKernelGGL(k_entry_sf, dim3((unsigned)(a.n * chunks)), dim3(256), 0, s, a);
+  }
   return hipGetLastError();
 }
 
@@ -673,7 +886,7 @@ hipError_t launch_map(const MapArgs& a, int64_t nbricks, hipStream_t s) {
 
 hipError_t launch_hydro(const HydroArgs& a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_hydro, dim3(a.n), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_hydro, dim3((unsigned)a.n), dim3(kHydroThreads), 0, s, a);
   return hipGetLastError();
 }
 

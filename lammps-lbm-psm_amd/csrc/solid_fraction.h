@@ -17,6 +17,8 @@
 // against the oracle's point-by-point count).
 #pragma once
 
+#include <math.h>
+
 #if defined(__HIPCC__)
 #define LBM_HD __host__ __device__ __forceinline__
 #else
@@ -205,6 +207,84 @@ LBM_HD int sf_count_columns(int dim, double dx, double dy, double dz, double rP_
   if (axis == 2) return sf_count_columns_axis<2>(lane, loop, col, rSq);
   if (axis == 0) return sf_count_columns_axis<0>(lane, loop, col, rSq);
   return sf_count_columns_axis<1>(lane, loop, col, rSq);
+}
+
+// ---------------------------------------------------------------------------
+// Count by lines (the 3D map kernel's form).
+//
+// Fix the x and y sub-point offsets (a, b) of a column of cells (i, j) and
+// run along z over every cell k of a zone and every slice c: the reference
+// tests fl(s + Z_p) < r^2 with s = fl(X_a + Y_b) and Z_p the squared z
+// offset of point p = 10 k + c.  The offsets fl(fl(k - z) + delta_c) grow
+// with p, so Z_p falls and then rises (a "V"), and the passing points form
+// one interval around the minimum.  Two binary searches find it; every
+// cell of the column then receives the length of its overlap.
+// ---------------------------------------------------------------------------
+
+// First index in [lo, hi] that passes, given that passing is monotone
+// non-decreasing on [lo, hi] and hi passes.
+LBM_HD int sf_first_pass(const double* Z, int lo, int hi, double s, double rSq) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (s + Z[mid] < rSq) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// Last index in [lo, hi] that passes, given that passing is monotone
+// non-increasing on [lo, hi] and lo passes.
+LBM_HD int sf_last_pass(const double* Z, int lo, int hi, double s, double rSq) {
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (s + Z[mid] < rSq) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// The interval of passing points of one line.  The points' z offsets are
+// t_p ~ t_0 + p/10, so the real-arithmetic chord ends |t| < sqrt(r^2 - s)
+// predict both ends; each prediction is confirmed by its two exact tests
+// (the end passes, its outer neighbour fails) and otherwise the binary
+// search decides.  Returns false for an empty line.
+LBM_HD bool sf_line_interval(const double* Z, int P, int pstar, double t0, double s, double rSq,
+                             int& lo, int& hi) {
+  if (!(s + Z[pstar] < rSq)) return false;
+  const float h = sqrtf((float)(rSq - s));
+  const float x0 = (float)t0;
+  int pe = (int)floorf((-h - x0) * 10.0f) + 1;
+  pe = pe < 0 ? 0 : (pe > pstar ? pstar : pe);
+  if (s + Z[pe] < rSq && (pe == 0 || !(s + Z[pe - 1] < rSq))) lo = pe;
+  else lo = sf_first_pass(Z, 0, pstar, s, rSq);
+  pe = (int)floorf((h - x0) * 10.0f);
+  pe = pe < pstar ? pstar : (pe > P - 1 ? P - 1 : pe);
+  if (s + Z[pe] < rSq && (pe == P - 1 || !(s + Z[pe + 1] < rSq))) hi = pe;
+  else hi = sf_last_pass(Z, pstar, P - 1, s, rSq);
+  return true;
+}
+
+// Index of the smallest square along a line (first one among equals).
+LBM_HD int sf_line_min(const double* Z, int P) {
+  int m = 0;
+  for (int p = 1; p < P; ++p)
+    if (Z[p] < Z[m]) m = p;
+  return m;
+}
+
+// Cells [klo, khi] of a line receive its passing points [lo, hi]: the two
+// end cells directly (part), the cells in between 10 each (diff, summed
+// along the line afterwards).
+LBM_HD void sf_line_split(int lo, int hi, int& klo, int& khi, int& nlo, int& nhi) {
+  klo = lo / kSfSlices;
+  khi = hi / kSfSlices;
+  if (klo == khi) {
+    nlo = hi - lo + 1;
+    nhi = 0;
+  } else {
+    nlo = kSfSlices * klo + kSfSlices - lo;
+    nhi = hi - kSfSlices * khi + 1;
+  }
 }
 
 // fraction * count, :120-124 and :168.

@@ -34,6 +34,15 @@ def _hip():
     return _HIP
 
 
+def device_count() -> int:
+    """HIP devices visible to the library's runtime (0 when none)."""
+    hip = _hip()
+    hip.hipGetDeviceCount.restype = C.c_int
+    hip.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
+    n = C.c_int(0)
+    return n.value if hip.hipGetDeviceCount(C.byref(n)) == 0 else 0
+
+
 class DeviceArray:
     """A copy of a numpy array in device memory (freed on close / GC)."""
 

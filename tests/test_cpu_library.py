@@ -75,8 +75,8 @@ def test_config_rejects_what_the_fix_rejects():
 
 
 def test_create_without_gpu_reports_nodevice():
-    import torch
-    if torch.cuda.is_available():
+    from lbmpsm.devmem import device_count
+    if device_count() > 0:
         pytest.skip("a GPU is present")
     cfg = lbmpsm.lattice_units_config(3, (8, 8, 8), (1, 1, 1), 0.7)
     with pytest.raises(lbmpsm.LbmPsmError) as e:
@@ -126,6 +126,31 @@ def test_library_solid_fraction_equals_oracle(sfcheck, dim):
             assert sfcheck.sf_split(dim, hs, *args, groups) == expect
         for axis in (-1, 0, 1, 2):
             assert sfcheck.sf_columns(dim, hs, *args, axis) == expect
+
+
+def test_line_form_counts_whole_zones(sfcheck):
+    """The 3D map kernel's line form (one interval per z line of sub-points)
+    gives every counted cell of whole sphere zones the per-cell count, which
+    test_library_solid_fraction_equals_oracle ties to the oracle."""
+    sfcheck.sf_lines_plane.restype = C.c_int
+    sfcheck.sf_lines_plane.argtypes = [C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.c_double, C.c_double, C.c_double, C.c_double]
+    hs = math.sqrt(3.1) / 2.0
+    rng = np.random.default_rng(31)
+    planes = 0
+    for trial in range(60):
+        r = rng.uniform(0.6, 11.0)
+        c = rng.uniform(20.0, 21.0, 3)
+        if trial % 5 == 0:
+            c = np.round(c * 4) / 4  # centres on sub-grid-aligned values
+        lo = (c - r).astype(int) - 2
+        hi = (c + r).astype(int) + 2
+        for i in range(lo[0], hi[0]):
+            bad = sfcheck.sf_lines_plane(hs, int(i), int(lo[1]), int(lo[2]), int(hi[1] - lo[1]),
+                                         int(hi[2] - lo[2]), c[0], c[1], c[2], r)
+            assert bad == 0, (trial, i)
+            planes += 1
+    assert planes > 600
 
 
 def test_ghost_images_corner():
