@@ -268,7 +268,9 @@ __device__ __forceinline__ int bulk_wave() {
   const int nb = (int)gridDim.x, per = nb >> 3, rem = nb & 7, x = b & 7;
   b = x * per + min(x, rem) + (b >> 3);
 #endif
-  return b * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+  // wave-uniform by construction; readfirstlane lets the compiler keep the
+  // row set-up in scalar registers and address with scalar bases
+  return __builtin_amdgcn_readfirstlane(b * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
 }
 
 // Collision of one node whose post-stream populations are in f[] (FP64).
