@@ -156,6 +156,17 @@ int lbmpsm_get_hydro(lbmpsm_ctx* ctx, int n, double* F, double* T, double* S);
 int lbmpsm_get_populations(lbmpsm_ctx* ctx, double* f);
 int lbmpsm_set_populations(lbmpsm_ctx* ctx, const double* f);
 
+/* The same populations in the reference's full lattice layout, the vector
+ * that LBMPSMLattice::getVector_f_curr returns on one rank and that
+ * `fix lbm-psm-restart` writes and reads
+ * (src/fix_LBM_PSM_writeReadRestart.cpp:80-103): [nx+2][ny+2][nz+2][q]
+ * (3D) or [nx+2][ny+2][1][q] (2D) with one halo node on both sides of each
+ * axis.  On non-periodic axes the halo is the lattice's own; on periodic
+ * axes get writes the periodic image (what the reference's halo refresh
+ * puts there before every step) and set ignores it. */
+int lbmpsm_get_populations_ref(lbmpsm_ctx* ctx, double* f);
+int lbmpsm_set_populations_ref(lbmpsm_ctx* ctx, const double* f);
+
 /* rho [nx][ny][nz] and u [nx][ny][nz][3] of the last step run with
  * LBMPSM_STEP_STORE_MACROS (reference get_rho_reference/get_u_reference,
  * src/LBM_PSM_lattice.cpp:214-222, interior nodes). */

@@ -875,6 +875,37 @@ int lbmpsm_set_populations(lbmpsm_ctx* c, const double* f) {
   return LBMPSM_OK;
 }
 
+static size_t ref_count(const lbmpsm_ctx* c) {
+  const Geom& g = c->g;
+  return (size_t)(g.nx + 2) * (g.ny + 2) * (g.dim == 3 ? g.nz + 2 : 1) * g.q;
+}
+
+int lbmpsm_get_populations_ref(lbmpsm_ctx* c, double* f) {
+  if (!c || !f) return LBMPSM_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t count = ref_count(c);
+  int rc = ensure_scratch(c, count * sizeof(double));
+  if (rc) return rc;
+  CollideArgs a = make_args(c, 0);
+  HIPCHK(c, launch_finish_ref(c->cfg.storage, a, c->d_scratch, c->stream));
+  HIPCHK(c, hipMemcpyAsync(f, c->d_scratch, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LBMPSM_OK;
+}
+
+int lbmpsm_set_populations_ref(lbmpsm_ctx* c, const double* f) {
+  if (!c || !f) return LBMPSM_ERR_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t count = ref_count(c);
+  int rc = ensure_scratch(c, count * sizeof(double));
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->d_scratch, f, count * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, launch_set_populations_ref(c->cfg.storage, c->f[c->cur], c->g, c->d_scratch, c->rho0, c->scale, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->poststream = 1;
+  return LBMPSM_OK;
+}
+
 int lbmpsm_get_macros(lbmpsm_ctx* c, double* rho, double* u) {
   if (!c) return LBMPSM_ERR_ARG;
   if (!c->have_macros) return fail(c, LBMPSM_ERR_STATE, "no step stored macroscopic fields");

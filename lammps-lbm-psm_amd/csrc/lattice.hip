@@ -574,15 +574,12 @@ __device__ __forceinline__ void gather(const CollideArgs& a, int i, int j, int k
   gather_row<DIM, Q, ST, true>(a, r, c, f);
 }
 
-// Boundary: nodes on at least one face.  Thread t walks the rectangle of
-// face `fi`; a node shared with an earlier face is left to that face's
-// threads, so each boundary node is updated once, with all its faces
-// applied in table order.
-template <int DIM, int Q, int ST>
-// One launch covers every face: consecutive block ranges belong to faces in
+// Boundary: nodes on at least one face, each updated once with all its
+// faces applied in table order.  One launch covers every face: consecutive block ranges belong to faces in
 // application order.  A node shared by several faces is handled by the
 // first face that holds it; the faces only read the previous state, so
 // their order of execution does not matter.
+template <int DIM, int Q, int ST>
 __global__ __launch_bounds__(64) void k_collide_faces(const CollideArgs a) {
   int fi = 0;
   int64_t b0 = 0;
@@ -642,6 +639,63 @@ __global__ __launch_bounds__(256) void k_finish(const CollideArgs a, double* out
   Unroll<0, Q>::run([&]<int q>() {
     if constexpr (ST == ST_F64) out[o + q] = f[q];
     else out[o + q] = (double)f[q] + VelSet<Q>::w[q] * a.rho0;
+  });
+}
+
+// The reference's own lattice layout (src/LBM_PSM_lattice.h:88-90): one halo
+// node on both sides of every axis of the active dimension, extents nx+2,
+// ny+2 and nz+2 (1 in 2D), populations fastest.  A non-periodic axis stores
+// its halo, so its reference coordinate is the stored one; on a periodic
+// axis the halo positions show the periodic image, as the reference's halo
+// refresh fills them.
+__device__ __forceinline__ int ref_extent_axis(const Geom& g, int axis) {
+  if (axis == 2 && g.dim == 2) return 1;
+  return (axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz)) + 2;
+}
+__device__ __forceinline__ int ref_to_stored(const Geom& g, int axis, int R, bool& image) {
+  image = false;
+  if ((axis == 2 && g.dim == 2) || !g.per[axis]) return R;
+  const int n = axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
+  image = R == 0 || R == n + 1;
+  const int s = R - 1;
+  return s < 0 ? s + n : (s >= n ? s - n : s);
+}
+
+template <int DIM, int Q, int ST>
+__global__ __launch_bounds__(256) void k_finish_ref(const CollideArgs a, double* out) {
+  const Geom& g = a.g;
+  const int ex = ref_extent_axis(g, 0), ey = ref_extent_axis(g, 1), ez = ref_extent_axis(g, 2);
+  const int64_t R = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (R >= (int64_t)ex * ey * ez) return;
+  const int K = (int)(R % ez), J = (int)((R / ez) % ey), I = (int)(R / ((int64_t)ey * ez));
+  bool m0, m1, m2;
+  const int i = ref_to_stored(g, 0, I, m0), j = ref_to_stored(g, 1, J, m1);
+  const int k = ref_to_stored(g, 2, K, m2);
+  typename StoreT<ST>::C f[Q];
+  gather<DIM, Q, ST>(a, i, j, k, f);
+  Unroll<0, Q>::run([&]<int q>() {
+    if constexpr (ST == ST_F64) out[R * Q + q] = f[q];
+    else out[R * Q + q] = (double)f[q] + VelSet<Q>::w[q] * a.rho0;
+  });
+}
+
+template <int Q, int ST>
+__global__ __launch_bounds__(256) void k_set_pop_ref(typename StoreT<ST>::T* f, Geom g,
+                                                     const double* in, double rho0, float scale) {
+  const int ex = ref_extent_axis(g, 0), ey = ref_extent_axis(g, 1), ez = ref_extent_axis(g, 2);
+  const int64_t R = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (R >= (int64_t)ex * ey * ez) return;
+  const int K = (int)(R % ez), J = (int)((R / ez) % ey), I = (int)(R / ((int64_t)ey * ez));
+  bool m0, m1, m2;
+  const int i = ref_to_stored(g, 0, I, m0), j = ref_to_stored(g, 1, J, m1);
+  const int k = ref_to_stored(g, 2, K, m2);
+  if (m0 || m1 || m2) return;  // periodic halo positions: images of stored nodes
+  const int64_t n = ((int64_t)i * g.sy + j) * g.sz + k;
+  Unroll<0, Q>::run([&]<int q>() {
+    const double v = in[R * Q + q];
+    if constexpr (ST == ST_F64) f[q * g.plane + n] = v;
+    else if constexpr (ST == ST_F32) f[q * g.plane + n] = (float)(v - VelSet<Q>::w[q] * rho0);
+    else f[q * g.plane + n] = __float2half_rn((float)(v - VelSet<Q>::w[q] * rho0) * scale);
   });
 }
 
@@ -766,6 +820,46 @@ static hipError_t finish_dim(int storage, const CollideArgs& a, double* out, hip
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+static inline int64_t ref_nodes(const Geom& g) {
+  return (int64_t)(g.nx + 2) * (g.ny + 2) * (g.dim == 3 ? g.nz + 2 : 1);
+}
+
+template <int DIM, int Q>
+static hipError_t finish_ref_dim(int storage, const CollideArgs& a, double* out, hipStream_t s) {
+  dim3 grid((unsigned)((ref_nodes(a.g) + 255) / 256));
+  switch (storage) {
+    case ST_F64: hipLaunchKernelGGL((k_finish_ref<DIM, Q, ST_F64>), grid, dim3(256), 0, s, a, out); break;
+    case ST_F32: hipLaunchKernelGGL((k_finish_ref<DIM, Q, ST_F32>), grid, dim3(256), 0, s, a, out); break;
+    case ST_F16: hipLaunchKernelGGL((k_finish_ref<DIM, Q, ST_F16>), grid, dim3(256), 0, s, a, out); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_finish_ref(int storage, const CollideArgs& a, double* out, hipStream_t s) {
+  return a.g.dim == 3 ? finish_ref_dim<3, 19>(storage, a, out, s)
+                      : finish_ref_dim<2, 9>(storage, a, out, s);
+}
+
+template <int Q>
+static hipError_t set_ref_q(int storage, void* f, const Geom& g, const double* in, double rho0,
+                            float scale, hipStream_t s) {
+  dim3 grid((unsigned)((ref_nodes(g) + 255) / 256));
+  switch (storage) {
+    case ST_F64: hipLaunchKernelGGL((k_set_pop_ref<Q, ST_F64>), grid, dim3(256), 0, s, (double*)f, g, in, rho0, scale); break;
+    case ST_F32: hipLaunchKernelGGL((k_set_pop_ref<Q, ST_F32>), grid, dim3(256), 0, s, (float*)f, g, in, rho0, scale); break;
+    case ST_F16: hipLaunchKernelGGL((k_set_pop_ref<Q, ST_F16>), grid, dim3(256), 0, s, (__half*)f, g, in, rho0, scale); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_set_populations_ref(int storage, void* f, const Geom& g, const double* in,
+                                      double rho0, float scale, hipStream_t s) {
+  return g.dim == 3 ? set_ref_q<19>(storage, f, g, in, rho0, scale, s)
+                    : set_ref_q<9>(storage, f, g, in, rho0, scale, s);
 }
 
 hipError_t launch_finish(int storage, const CollideArgs& a, double* out, hipStream_t s) {

@@ -130,6 +130,12 @@ hipError_t launch_finish(int storage, const CollideArgs& a, double* out, hipStre
 // Interior reference-order FP64 populations (post-stream) into storage form.
 hipError_t launch_set_populations(int storage, void* f, const Geom& g, const double* in,
                                   double rho0, float scale, hipStream_t s);
+// The same in the reference's full halo'd layout ([nx+2][ny+2][nz+2 or 1][q]):
+// periodic halo positions show their periodic image on output and are
+// skipped on input.
+hipError_t launch_finish_ref(int storage, const CollideArgs& a, double* out, hipStream_t s);
+hipError_t launch_set_populations_ref(int storage, void* f, const Geom& g, const double* in,
+                                      double rho0, float scale, hipStream_t s);
 // Mass of the post-stream state: per-block partial sums into `partials`.
 hipError_t launch_mass(int storage, const CollideArgs& a, double* partials, int nblocks,
                        hipStream_t s);

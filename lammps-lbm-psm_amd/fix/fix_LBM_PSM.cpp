@@ -38,7 +38,8 @@ constexpr int kHydroCols = 6;
 fix_LBM_PSM::fix_LBM_PSM(LAMMPS* lmp, int narg, char** arg)
     : Fix(lmp, narg, arg),
       Nlc(0), lc(0.0), rho(0.0), nu(0.0), Re(0.0), tau(0.7), storage(LBMPSM_STORE_F64),
-      device(0), bc_keyword(LBMPSM_BC_NONE), ctx(nullptr), hydrodynamicInteractions(nullptr) {
+      device(0), bc_keyword(LBMPSM_BC_NONE), store_macros(0), ctx(nullptr),
+      hydrodynamicInteractions(nullptr) {
   const char* illegal = "Illegal fix lbm-psm command";
   if (narg < 15) error->all(FLERR, illegal);
 
@@ -221,7 +222,7 @@ void fix_LBM_PSM::post_force(int vflag) {
   const double* v = n ? atom->v[0] : nullptr;
   const double* om = n ? atom->omega[0] : nullptr;
   check(lbmpsm_set_particles(ctx, n, tag, x, v, om, atom->radius), "lbmpsm_set_particles");
-  check(lbmpsm_step(ctx, 0), "lbmpsm_step");
+  check(lbmpsm_step(ctx, store_macros ? LBMPSM_STEP_STORE_MACROS : 0), "lbmpsm_step");
   Fbuf.resize((size_t)n * 3);
   Tbuf.resize((size_t)n * 3);
   Sbuf.resize((size_t)n * 6);

@@ -51,6 +51,9 @@ class fix_LBM_PSM : public Fix {
   // step (src/fix_LBM_PSM_BC.cpp:151-402 applies it after collide-stream).
   void set_boundary(int keyword);
 
+  // Keep rho and u of every LBM step (for `fix lbm-psm-vti`).
+  void request_macros() { store_macros = 1; }
+
   void grow_arrays(int) override;
   void copy_arrays(int, int, int) override;
   int pack_exchange(int, double*) override;
@@ -65,6 +68,7 @@ class fix_LBM_PSM : public Fix {
   double F_ext[3];
   int storage, device;
   int bc_keyword;
+  int store_macros;
 
   lbmpsm_config cfg;
   lbmpsm_ctx* ctx;

@@ -1,11 +1,11 @@
 // A minimal LAMMPS time-step loop around the lbmpsm fixes, with a C ABI for
-// the tests.  It mirrors what LAMMPS does around post_force for an
-// atom_style sphere run with no pair style: fixes are created from their
+// the tests.  It mirrors what LAMMPS does around the force computation for
+// an atom_style sphere run with no pair style: fixes are created from their
 // command words in deck order, init() runs once per `run`, and every step
 // sets the atoms (locals then ghost images, as the test's neighbour build
-// hands them over), clears f and torque, calls post_force of every fix in
-// order and lets reverse_comm add each ghost's force and torque to its
-// owner.  Test infrastructure only.
+// hands them over), clears f and torque, calls pre_force and then
+// post_force of every fix in order and lets reverse_comm add each ghost's
+// force and torque to its owner.  Test infrastructure only.
 #include <cstring>
 #include <memory>
 #include <sstream>
@@ -14,6 +14,8 @@
 
 #include "../fix_LBM_PSM.h"
 #include "../fix_LBM_PSM_BC.h"
+#include "../fix_LBM_PSM_writeReadRestart.h"
+#include "../fix_LBM_PSM_writeVTI.h"
 #include "lammps_api.h"
 
 using namespace LAMMPS_NS;
@@ -122,6 +124,10 @@ int mock_fix(mock_lmp* m, const char* line) {
       f.reset(new fix_LBM_PSM(&m->lmp, narg, argv.data()));
     else if (kept[2] == "lbm-psm-bc")
       f.reset(new fix_LBM_PSM_BC(&m->lmp, narg, argv.data()));
+    else if (kept[2] == "lbm-psm-restart")
+      f.reset(new LBMPSMWriteReadRestart(&m->lmp, narg, argv.data()));
+    else if (kept[2] == "lbm-psm-vti")
+      f.reset(new WriteVTI(&m->lmp, narg, argv.data()));
     else
       throw std::runtime_error("Unrecognized fix style '" + kept[2] + "'");
     if (m->lmp.atom->callback_requested) m->callbacks.push_back(f.get());
@@ -161,6 +167,8 @@ int mock_step(mock_lmp* m, int nlocal, int nghost, const int* tag, const double*
     }
     m->owner.assign(owner, owner + nghost);
     m->lmp.update->ntimestep += 1;
+    for (auto& f : m->fixes)
+      if (f->setmask() & FixConst::PRE_FORCE) f->pre_force(1);
     for (auto& f : m->fixes)
       if (f->setmask() & FixConst::POST_FORCE) f->post_force(1);
     if (nlocal) {

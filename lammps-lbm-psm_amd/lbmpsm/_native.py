@@ -110,6 +110,8 @@ def _load() -> C.CDLL:
         "lbmpsm_get_hydro": (C.c_int, [P, C.c_int, dp, dp, dp]),
         "lbmpsm_get_populations": (C.c_int, [P, dp]),
         "lbmpsm_set_populations": (C.c_int, [P, dp]),
+        "lbmpsm_get_populations_ref": (C.c_int, [P, dp]),
+        "lbmpsm_set_populations_ref": (C.c_int, [P, dp]),
         "lbmpsm_get_macros": (C.c_int, [P, dp, dp]),
         "lbmpsm_get_slots": (C.c_int, [P, ip, dp, dp, dp]),
         "lbmpsm_mass": (C.c_int, [P, dp]),

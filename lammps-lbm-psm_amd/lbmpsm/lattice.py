@@ -188,6 +188,21 @@ class Lattice:
         assert f.size == self.nodes * self.q
         self._check(N.LIB.lbmpsm_set_populations(self._h, N.dptr(f)))
 
+    def ref_shape(self):
+        nx, ny, nz = self.shape
+        return (nx + 2, ny + 2, nz + 2 if self.cfg.dim == 3 else 1, self.q)
+
+    def populations_ref(self):
+        """Populations in the reference's halo'd layout (restart files)."""
+        f = np.empty(self.ref_shape())
+        self._check(N.LIB.lbmpsm_get_populations_ref(self._h, N.dptr(f)))
+        return f
+
+    def set_populations_ref(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert f.shape == self.ref_shape()
+        self._check(N.LIB.lbmpsm_set_populations_ref(self._h, N.dptr(f)))
+
     def macros(self):
         nx, ny, nz = self.shape
         rho = np.empty((nx, ny, nz))
