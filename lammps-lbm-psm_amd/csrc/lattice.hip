@@ -437,6 +437,45 @@ template <int DIM, int Q, int ST>
 __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, unsigned rb,
                                              unsigned uc);
 
+// Odd part of the FP32 equilibrium, 3 w_q (e_q . p) with p = rho u, formed
+// as a signed sum of A = 3 w_axis p per axis (w_axis = 1/18 in D3Q19, 1/9 in
+// D2Q9); a diagonal weight is w_axis/2 (D3Q19) or w_axis/4 (D2Q9), a power
+// of two.  Two roundings would bias the momentum that every step hands to
+// the flow, and over thousands of steps under a small body force either one
+// moves the drag by ~1e-5:
+//  * the float weight alone scales every equilibrium's momentum by one
+//    constant factor (1 + 7.4e-9 for w = 1/18), a force omega * 7.4e-9 *
+//    rho u per step; 3 w_axis is therefore applied as two floats, hi + lo;
+//  * rounding p = j + F/2 to float adds the same error, (F/2 mod ulp(j)), at
+//    every node whose j lies in one binade, a fixed fraction of F (up to
+//    ~1e-5 of it); the half force is instead folded into the last fused
+//    multiply-add, whose single rounding falls on the exact product j * hi.
+template <int Q>
+struct OddK {
+  static constexpr double k = 3.0 * VelSet<Q>::w[1];
+  static constexpr float hi = (float)k;
+  static constexpr float lo = (float)(k - (double)hi);
+};
+// A = 3 w_axis (j + F/2), with hf = 3 w_axis F/2 (a per-launch constant).
+template <int Q>
+__device__ __forceinline__ float odd_axis_j(float j, float hf) {
+  return __builtin_fmaf(j, OddK<Q>::hi, __builtin_fmaf(j, OddK<Q>::lo, hf));
+}
+template <int Q, int q>
+__device__ __forceinline__ float eq_odd(float Ax, float Ay, float Az) {
+  constexpr int cx = VelSet<Q>::ex[q], cy = VelSet<Q>::ey[q], cz = VelSet<Q>::ez[q];
+  constexpr int n2 = cx * cx + cy * cy + cz * cz;
+  static_assert(n2 <= 2, "velocity sets with |e|^2 <= 2 only");
+  if constexpr (n2 == 0) {
+    return 0.0f;
+  } else if constexpr (n2 == 1) {
+    return edot<Q, q>(Ax, Ay, Az);
+  } else {
+    constexpr float r = (float)(VelSet<Q>::w[q] / VelSet<Q>::w[1]);
+    return r * edot<Q, q>(Ax, Ay, Az);
+  }
+}
+
 template <int DIM, int Q, int ST>
 __global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_lowp(const CollideArgs a) {
   const Geom& g = a.g;
@@ -462,6 +501,8 @@ __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, uns
   const float omega = (float)a.omega;
   const float rho0 = (float)a.rho0;
   const float F0 = (float)a.F[0], F1 = (float)a.F[1], F2 = (float)a.F[2];
+  const float hF0 = (float)(0.5 * OddK<Q>::k * a.F[0]), hF1 = (float)(0.5 * OddK<Q>::k * a.F[1]),
+              hF2 = (float)(0.5 * OddK<Q>::k * a.F[2]);
   const float gp = (float)a.guo_pref;
   {
     const unsigned n = rb + uc;
@@ -479,13 +520,17 @@ __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, uns
     }
     const float usq = 1.5f * (ux * ux + uy * uy + uz * uz);
     const uint8_t m = a.mask[n];
-    // g_eq = w (drho + rho (3 e.u + 4.5 (e.u)^2 - 1.5 u^2)); mixed waves run
-    // the particle form for all lanes (wave-uniform branch, whole-wave stores)
+    // g_eq = w (drho + rho (4.5 (e.u)^2 - 1.5 u^2)) + 3 w e.(rho u), with
+    // rho u = j + F/2 (u itself only enters terms even in e); mixed waves
+    // run the particle form for all lanes
+    // (wave-uniform branch, whole-wave stores)
+    const float Ax = odd_axis_j<Q>(jx, hF0), Ay = odd_axis_j<Q>(jy, hF1),
+                Az = odd_axis_j<Q>(jz, hF2);
     if (!__any(m != 0)) {
       Unroll<0, Q>::run([&]<int q>() {
         constexpr float w = (float)VelSet<Q>::w[q];
         const float eu = edot<Q, q>(ux, uy, uz);
-        const float geq = w * (drho + rho * (3.0f * eu + 4.5f * eu * eu - usq));
+        const float geq = w * (drho + rho * (4.5f * eu * eu - usq)) + eq_odd<Q, q>(Ax, Ay, Az);
         float out = f[q] + omega * (geq - f[q]);
         if (a.guo) {
           constexpr float ex = VelSet<Q>::ex[q], ey = VelSet<Q>::ey[q], ez = VelSet<Q>::ez[q];
@@ -524,8 +569,10 @@ __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, uns
         constexpr int cx = VelSet<Q>::ex[q], cy = VelSet<Q>::ey[q], cz = VelSet<Q>::ez[q];
         const float gi = f[q];
         const float eu = edot<Q, q>(ux, uy, uz);
-        const float geq = w * (drho + rho * (3.0f * eu + 4.5f * eu * eu - usq));
+        const float geq = w * (drho + rho * (4.5f * eu * eu - usq)) + eq_odd<Q, q>(Ax, Ay, Az);
         float sc1 = 0.f, sc2 = 0.f;
+        // the solid equilibrium keeps the plain float weight: its momentum
+        // bias, 7.4e-9 rho u_s, is ~1e-8 of the node's exchange B rho (u_s - u)
         if (o1) {
           const float e1 = edot<Q, q>(us1[0], us1[1], us1[2]);
           const float gs = w * (drho + rho * (3.0f * e1 + 4.5f * e1 * e1 - u1sq));

@@ -8,13 +8,10 @@
   d ~ U[10, 20] (seeded RSA, largest first), tau in {0.6, 0.65, 0.8}, 3000
   steps in FP64, FP32 and FP16 storage: per-sphere drag, torque and
   stresslet error of FP32 and FP16 against FP64, relative to the largest
-  value, reported per tau.  FP32 keeps ~7 significant digits of each stored
-  deviation f - w rho0; in this dense, slow suspension the non-equilibrium
-  part that carries the drag is a small difference of such values, and 3000
-  steps of rounding leave errors of ~1e-5 (largest at tau 0.6).  The
-  asserted bounds, 5e-5 (FP32) and 2e-2 (FP16), are the tolerance of the
-  storage forms on this workload; the FP32 bound of 1e-5 applies to the
-  192^3 moving-sphere workload (tests/test_gpu_precision.py, bench.py).
+  value.  FP32 within 1e-5 (measured 0.6e-6 to 3.6e-6: the flow here is
+  driven by a body force of 5e-6 per node and step, so any rounding bias of
+  the momentum a collision hands on shows in the drag; the FP32 equilibrium
+  is formed to carry none), FP16 within 2e-2 (measured up to 2e-3).
 """
 import numpy as np
 import pytest
@@ -91,7 +88,7 @@ def test_dense_polydisperse_reduced_storage(tau):
         lat.close()
         print(f"  {storage}: mean drag {np.mean(Fo[:, 0]):.6e}", flush=True)
     F64, T64, S64 = res["f64"]
-    for storage, bound in (("f32", 5e-5), ("f16", 2e-2)):
+    for storage, bound in (("f32", 1e-5), ("f16", 2e-2)):
         F, T, S = res[storage]
         eF = np.max(np.linalg.norm(F - F64, axis=1)) / np.max(np.linalg.norm(F64, axis=1))
         eT = np.max(np.linalg.norm(T - T64, axis=1)) / np.max(np.linalg.norm(T64, axis=1))
