@@ -284,7 +284,7 @@ int ensure_particles(lbmpsm_ctx* c, int n) {
   // earlier steps may still read these buffers
   HIPCHK(c, hipStreamSynchronize(c->stream));
   int cap = std::max(n, std::max(64, P.cap * 2));
-  void* old[] = {P.raw, P.xl, P.kin, P.zone, P.rzone, P.out};
+  void* old[] = {P.raw, P.xl, P.kin, P.zone, P.rzone, P.out, P.hpart};
   for (void* p : old) dev_free(p);
   int rc = 0;
   rc |= dev_alloc(c, &P.raw, 84 * (size_t)cap);
@@ -293,6 +293,7 @@ int ensure_particles(lbmpsm_ctx* c, int n) {
   rc |= dev_alloc(c, &P.zone, 6 * (size_t)cap);
   rc |= dev_alloc(c, &P.rzone, 6 * (size_t)cap);
   rc |= dev_alloc(c, &P.out, 12 * (size_t)cap);
+  rc |= dev_alloc(c, &P.hpart, (size_t)kHydroGroups * 12 * (size_t)cap);
   if (rc) return LBMPSM_ERR_NOMEM;
   P.cap = cap;
   if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -658,7 +659,7 @@ int lbmpsm_destroy(lbmpsm_ctx* c) {
   dev_free(c->rho);
   dev_free(c->u);
   ParticleDev& P = c->part;
-  void* pp[] = {P.raw, P.xl, P.kin, P.zone, P.rzone, P.out};
+  void* pp[] = {P.raw, P.xl, P.kin, P.zone, P.rzone, P.out, P.hpart};
   for (void* p : pp) dev_free(p);
   BrickDev& B = c->br;
   void* bb[] = {B.count, B.list};
@@ -818,6 +819,7 @@ int lbmpsm_step(lbmpsm_ctx* c, int flags) {
   h.Fh = c->slots.Fh;
   h.forceFactor = c->cfg.force_factor;
   h.torqueFactor = c->cfg.torque_factor;
+  h.part = c->part.hpart;
   h.out = c->part.out;
   if (h.n > 0) {
     ProfScope ps(c, "hydro_reduce");

@@ -68,7 +68,12 @@ template <int ST>
 __device__ __forceinline__ typename StoreT<ST>::C ld(const typename StoreT<ST>::T* p,
                                                       float inv_scale) {
   if constexpr (ST == ST_F16) {
+#if LBMPSM_NT_LOAD
+    const unsigned short b = __builtin_nontemporal_load((const unsigned short*)p);
+    return __half2float(__ushort_as_half(b)) * inv_scale;
+#else
     return __half2float(*p) * inv_scale;
+#endif
   } else {
     (void)inv_scale;
 #if LBMPSM_NT_LOAD
@@ -83,7 +88,11 @@ template <int ST>
 __device__ __forceinline__ void st(typename StoreT<ST>::T* p, typename StoreT<ST>::C v,
                                    float scale) {
   if constexpr (ST == ST_F16) {
+#if LBMPSM_NT_STORE
+    __builtin_nontemporal_store(__half_as_ushort(__float2half_rn(v * scale)), (unsigned short*)p);
+#else
     *p = __float2half_rn(v * scale);
+#endif
   } else {
     (void)scale;
 #if LBMPSM_NT_STORE

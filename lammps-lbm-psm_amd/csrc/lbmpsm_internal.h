@@ -54,6 +54,12 @@ struct CollideArgs {
   const Face* host_faces;  // the same table in host memory (launch sizes only)
 };
 
+// Workgroups per entry of the force/torque/stresslet reduction.
+#ifndef LBMPSM_HYDRO_GROUPS
+#define LBMPSM_HYDRO_GROUPS 4
+#endif
+constexpr int kHydroGroups = LBMPSM_HYDRO_GROUPS;
+
 // Particle list state on the device.  The raw LAMMPS arrays of one call
 // arrive in one copy: x[n][3], v[n][3], omega[n][3], radius[n] (doubles)
 // followed by tag[n] (ints), i.e. 84 bytes per entry.
@@ -67,6 +73,7 @@ struct ParticleDev {
   int* zone = nullptr;      // [n][6]  stored-coordinate map zone [lo,hi) x,y,z
   int* rzone = nullptr;     // [n][6]  stored-coordinate reduction zone
   double* out = nullptr;    // [n][12] F, T, S (physical units)
+  double* hpart = nullptr;  // [n][kHydroGroups][12] group sums of the reduction
   const double* x(int nn) const { return (const double*)raw; }
   const double* v(int nn) const { return (const double*)raw + 3 * (size_t)nn; }
   const double* w(int nn) const { return (const double*)raw + 6 * (size_t)nn; }
@@ -177,6 +184,7 @@ struct HydroArgs {
   const int* slot_tag;
   const double* Fh;
   double forceFactor, torqueFactor;
+  double* part;  // [n][kHydroGroups][12] group sums
   double* out;
 };
 hipError_t launch_hydro(const HydroArgs& a, hipStream_t s);

@@ -681,17 +681,31 @@ __global__ __launch_bounds__(256) LBMPSM_MAP_OCC void k_map(const MapArgs a) {
   if (S.d0 || S.d1) a.s.mask[n] = (uint8_t)((S.t0 > 0 ? 1 : 0) | (S.t1 > 0 ? 2 : 0));
 }
 
-// Force, torque and stresslet of one list entry per workgroup of 1024
-// threads (16 waves): each thread sums its nodes of the entry's reduction
-// zone, then the sums are combined in a fixed order - a shuffle tree within
-// each wave, then the 16 wave sums in wave order.  Terms as
-// src/LBM_PSM_exchangeParticleData.cpp:197-240: F += Fhyd ff,
-// T += (r x Fhyd) tf, S -= 1/2 (r_a F_b + r_b F_a) tf with r the node
-// offset from the entry's lattice position.
-constexpr int kHydroThreads = 1024;
+// Force, torque and stresslet of one list entry, split over kHydroGroups
+// workgroups of kHydroThreads threads.  The entry's reduction zone (nodes
+// in i, j, k order) is cut into kHydroGroups equal ranges, one per
+// workgroup; each thread sums its nodes of the range, kHydroU at a time so
+// that their loads are in flight together.  The sums are combined in a
+// fixed order: a shuffle tree within each wave, the wave sums in wave order,
+// then (k_hydro_sum) the group sums in group order.  Terms as src/LBM_PSM_exchangeParticleData.cpp:197-240:
+// F += Fhyd ff, T += (r x Fhyd) tf, S -= 1/2 (r_a F_b + r_b F_a) tf with r
+// the node offset from the entry's lattice position.
+#ifndef LBMPSM_HYDRO_THREADS
+#define LBMPSM_HYDRO_THREADS 256
+#endif
+#ifndef LBMPSM_HYDRO_GROUPS
+#define LBMPSM_HYDRO_GROUPS 4
+#endif
+#ifndef LBMPSM_HYDRO_U
+#define LBMPSM_HYDRO_U 4
+#endif
+constexpr int kHydroThreads = LBMPSM_HYDRO_THREADS;
+constexpr int kHydroU = LBMPSM_HYDRO_U;
+static_assert(kHydroThreads % 64 == 0 && kHydroThreads <= 1024, "whole waves");
 
 __global__ __launch_bounds__(kHydroThreads) void k_hydro(const HydroArgs a) {
-  const int p = blockIdx.x;
+  const int p = (int)blockIdx.x / kHydroGroups;
+  const int grp = (int)blockIdx.x % kHydroGroups;
   if (p >= a.n) return;
   const Geom& g = a.g;
   const int64_t plane = g.plane;
@@ -704,10 +718,11 @@ __global__ __launch_bounds__(kHydroThreads) void k_hydro(const HydroArgs a) {
   const int ni = z[1] - z[0], nj = z[3] - z[2], nk = z[5] - z[4];
   const int tot = (ni > 0 && nj > 0 && nk > 0) ? ni * nj * nk : 0;
   const int njk = nj * nk;
-  // U nodes per thread per pass so that their loads are in flight together;
+  const int chunk = (tot + kHydroGroups - 1) / kHydroGroups;
+  const int lo = grp * chunk, hi = min(tot, lo + chunk);
   // 32-bit index arithmetic (a zone holds far fewer than 2^31 nodes)
-  constexpr int U = 4;
-  for (int base = threadIdx.x; base < tot; base += kHydroThreads * U) {
+  constexpr int U = kHydroU;
+  for (int base = lo + (int)threadIdx.x; base < hi; base += kHydroThreads * U) {
     int nn[U], sl[U], ii[U], jj[U], kk[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -715,7 +730,7 @@ __global__ __launch_bounds__(kHydroThreads) void k_hydro(const HydroArgs a) {
       sl[u] = -1;
       nn[u] = 0;
       ii[u] = jj[u] = kk[u] = 0;
-      if (t < tot) {
+      if (t < hi) {
         const int li = t / njk;
         const int rem = t - li * njk;
         const int lj = rem / nk;
@@ -760,11 +775,26 @@ __global__ __launch_bounds__(kHydroThreads) void k_hydro(const HydroArgs a) {
   if (lane == 0)
     for (int c = 0; c < 12; ++c) wsum[wave][c] = acc[c];
   __syncthreads();
-  if (threadIdx.x < 12) {
+  if (threadIdx.x != 0) return;
+  double gs[12];
+  for (int c = 0; c < 12; ++c) {
     double s = 0.0;
-    for (int w = 0; w < kHydroThreads / 64; ++w) s += wsum[w][threadIdx.x];
-    a.out[12 * p + threadIdx.x] = s;
+    for (int w = 0; w < kHydroThreads / 64; ++w) s += wsum[w][c];
+    gs[c] = s;
   }
+  double* mine = a.part + ((size_t)p * kHydroGroups + grp) * 12;
+  for (int c = 0; c < 12; ++c) mine[c] = gs[c];
+}
+
+// Per entry and component: the group sums in group order.
+__global__ __launch_bounds__(256) void k_hydro_sum(const HydroArgs a) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 12 * a.n) return;
+  const int p = t / 12, c = t - 12 * p;
+  const double* all = a.part + (size_t)p * kHydroGroups * 12;
+  double s = 0.0;
+  for (int q = 0; q < kHydroGroups; ++q) s += all[q * 12 + c];
+  a.out[t] = s;
 }
 
 __device__ __forceinline__ bool interior_of(const Geom& g, int64_t n, int64_t& o) {
@@ -886,7 +916,8 @@ hipError_t launch_map(const MapArgs& a, int64_t nbricks, hipStream_t s) {
 
 hipError_t launch_hydro(const HydroArgs& a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_hydro, dim3((unsigned)a.n), dim3(kHydroThreads), 0, s, a);
+  hipLaunchKernelGGL(k_hydro, dim3((unsigned)a.n * kHydroGroups), dim3(kHydroThreads), 0, s, a);
+  hipLaunchKernelGGL(k_hydro_sum, grid_for(12 * (int64_t)a.n, 256), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
