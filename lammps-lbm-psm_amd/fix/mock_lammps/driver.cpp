@@ -16,12 +16,18 @@
 #include "../fix_LBM_PSM_BC.h"
 #include "../fix_LBM_PSM_writeReadRestart.h"
 #include "../fix_LBM_PSM_writeVTI.h"
+#include "../pair_GRM_LBDEM.h"
 #include "lammps_api.h"
 
 using namespace LAMMPS_NS;
 
 struct mock_lmp {
   LAMMPS lmp;
+  std::unique_ptr<PairLubricateGRMLBDEM> pair;
+  bool pair_ready = false;
+  NeighList nlist;
+  std::vector<int> ilist, numneigh, neigh;
+  std::vector<int*> firstneigh;
   std::vector<std::unique_ptr<Fix>> fixes;
   std::vector<Fix*> callbacks;                  // fixes with per-atom arrays
   std::vector<std::vector<std::string>> words;  // keeps argv strings alive
@@ -42,6 +48,7 @@ void grow_atoms(mock_lmp* m, int nmax) {
   mem->grow(a->f, nmax, 3, "atom:f");
   mem->grow(a->omega, nmax, 3, "atom:omega");
   mem->grow(a->torque, nmax, 3, "atom:torque");
+  mem->grow(a->type, nmax, "atom:type");
   a->nmax = nmax;
   for (Fix* f : m->callbacks) f->grow_arrays(nmax);
 }
@@ -98,6 +105,7 @@ void mock_close(mock_lmp* m) {
   mem->destroy(a->f);
   mem->destroy(a->omega);
   mem->destroy(a->torque);
+  mem->destroy(a->type);
   delete m;
 }
 
@@ -182,4 +190,177 @@ int mock_step(mock_lmp* m, int nlocal, int nghost, const int* tag, const double*
   });
 }
 
-}  // extern "C"
+// ---- pair_style lubricate/GRM/LBDEM -----------------------------------------
+namespace {
+std::vector<char*> split_words(mock_lmp* m, const char* line) {
+  std::istringstream is(line);
+  std::vector<std::string> w;
+  for (std::string s; is >> s;) w.push_back(s);
+  m->words.push_back(w);
+  std::vector<char*> argv;
+  for (auto& s : m->words.back()) argv.push_back(s.data());
+  return argv;
+}
+}  // namespace
+
+// Atom types, newton_pair, comm_modify vel, units (force->vxmu2f), and the
+// deform / wall fixes init_style looks for ("deform v|x", "wall 0|1").
+int mock_pair_env(mock_lmp* m, int ntypes, int newton_pair, int ghost_velocity, double vxmu2f,
+                  const char* extra_fix) {
+  return guarded(m, [&] {
+    m->lmp.atom->ntypes = ntypes;
+    m->lmp.force->newton_pair = newton_pair;
+    m->lmp.comm->ghost_velocity = ghost_velocity;
+    m->lmp.force->vxmu2f = vxmu2f;
+    if (extra_fix && *extra_fix) {
+      auto argv = split_words(m, extra_fix);
+      std::vector<char*> fa = {argv[0], argv[0], argv[0]};
+      const std::string kind = argv[0];
+      if (kind == "deform") {
+        auto* f = new FixDeform(&m->lmp, 3, fa.data());
+        f->style = (char*)"deform";
+        f->remapflag = std::string(argv[1]) == "v" ? Domain::V_REMAP : Domain::X_REMAP;
+        m->lmp.modify->add(f);
+        m->fixes.emplace_back(f);
+      } else if (kind == "wall") {
+        auto* f = new FixWall(&m->lmp, 3, fa.data());
+        f->style = (char*)"wall/lj93";
+        f->xflag = std::atoi(argv[1]);
+        m->lmp.modify->add(f);
+        m->fixes.emplace_back(f);
+      } else {
+        throw std::runtime_error("unknown extra fix " + kind);
+      }
+    }
+  });
+}
+
+// `pair_style lubricate/GRM/LBDEM args...` without the word "pair_style"
+int mock_pair_style(mock_lmp* m, const char* line) {
+  return guarded(m, [&] {
+    auto argv = split_words(m, line);
+    if (argv.empty() || std::string(argv[0]) != "lubricate/GRM/LBDEM")
+      throw std::runtime_error("Unrecognized pair style");
+    if (!m->pair) m->pair.reset(new PairLubricateGRMLBDEM(&m->lmp));
+    m->pair_ready = false;
+    m->pair->settings((int)argv.size() - 1, argv.data() + 1);
+  });
+}
+
+// `pair_coeff I J [cut_inner cut]` without the word "pair_coeff"
+int mock_pair_coeff(mock_lmp* m, const char* line) {
+  return guarded(m, [&] {
+    if (!m->pair) throw std::runtime_error("Pair_coeff command before pair_style is defined");
+    auto argv = split_words(m, line);
+    m->pair_ready = false;
+    m->pair->coeff((int)argv.size(), argv.data());
+  });
+}
+
+// One force evaluation on a given full neighbour list: atoms (locals then
+// ghosts) with their types, neigh holding numneigh[i] partner indices of
+// local i in turn.  Pair::init runs first when the settings changed: every
+// type pair must be set or mixable, cutsq = cut^2.  vflag 1 tallies the
+// global virial.  Returns the locals' f and torque and the virial.
+int mock_pair_compute(mock_lmp* m, int nlocal, int nghost, const int* type, const double* x,
+                      const double* v, const double* omega, const double* radius,
+                      const int* numneigh, const int* neigh, int vflag, double* f_out,
+                      double* t_out, double* virial_out) {
+  return guarded(m, [&] {
+    if (!m->pair) throw std::runtime_error("no pair style");
+    const int n = nlocal + nghost;
+    grow_atoms(m, n > 0 ? n : 1);
+    Atom* a = m->lmp.atom;
+    a->nlocal = nlocal;
+    a->nghost = nghost;
+    if (n) {
+      std::memcpy(a->type, type, sizeof(int) * n);
+      std::memcpy(a->radius, radius, sizeof(double) * n);
+      std::memcpy(a->x[0], x, sizeof(double) * 3 * n);
+      std::memcpy(a->v[0], v, sizeof(double) * 3 * n);
+      std::memcpy(a->omega[0], omega, sizeof(double) * 3 * n);
+      std::memset(a->f[0], 0, sizeof(double) * 3 * n);
+      std::memset(a->torque[0], 0, sizeof(double) * 3 * n);
+    }
+    PairLubricateGRMLBDEM* p = m->pair.get();
+    if (!m->pair_ready) {
+      p->init_style();
+      for (int i = 1; i <= a->ntypes; ++i)
+        for (int j = i; j <= a->ntypes; ++j) {
+          const double c = p->init_one(i, j);
+          p->cutsq[i][j] = p->cutsq[j][i] = c * c;
+        }
+      m->pair_ready = true;
+    }
+    int tot = 0;
+    for (int i = 0; i < nlocal; ++i) tot += numneigh[i];
+    m->ilist.resize(nlocal > 0 ? nlocal : 1);
+    m->numneigh.assign(numneigh, numneigh + nlocal);
+    m->neigh.assign(neigh, neigh + tot);
+    m->firstneigh.resize(n > 0 ? n : 1);
+    for (int i = 0, o = 0; i < nlocal; o += numneigh[i], ++i) {
+      m->ilist[i] = i;
+      m->firstneigh[i] = m->neigh.data() + o;
+    }
+    m->nlist.inum = nlocal;
+    m->nlist.ilist = m->ilist.data();
+    m->nlist.numneigh = m->numneigh.data();
+    m->nlist.firstneigh = m->firstneigh.data();
+    p->list = &m->nlist;
+    p->compute(0, vflag);
+    if (nlocal) {
+      std::memcpy(f_out, a->f[0], sizeof(double) * 3 * nlocal);
+      std::memcpy(t_out, a->torque[0], sizeof(double) * 3 * nlocal);
+    }
+    for (int c = 0; c < 6; ++c) virial_out[c] = p->virial[c];
+  });
+}
+
+// Neighbour request flags of the last init (NeighConst)
+int mock_pair_request(mock_lmp* m) { return m->lmp.neighbor->requested; }
+
+// write_restart / read_restart of the pair's settings and coefficients; the
+// read replaces the pair with a fresh instance built from the file
+int mock_pair_restart(mock_lmp* m, const char* path, int write) {
+  return guarded(m, [&] {
+    FILE* fp = std::fopen(path, write ? "wb" : "rb");
+    if (!fp) throw std::runtime_error(std::string("cannot open ") + path);
+    try {
+      if (write) {
+        if (!m->pair) throw std::runtime_error("no pair style");
+        m->pair->write_restart(fp);
+      } else {
+        m->pair.reset(new PairLubricateGRMLBDEM(&m->lmp));
+        m->pair->read_restart(fp);
+        m->pair_ready = false;
+      }
+    } catch (...) {
+      std::fclose(fp);
+      throw;
+    }
+    std::fclose(fp);
+  });
+}
+
+// pack_comm / unpack_comm round trip of ghost v and omega: packs the atoms
+// in list[n] and unpacks them into atoms first..first+n-1
+int mock_pair_comm(mock_lmp* m, int n, const int* list, int first, double* buf) {
+  return guarded(m, [&] {
+    if (!m->pair) throw std::runtime_error("no pair style");
+    std::vector<int> l(list, list + n);
+    const int per = m->pair->pack_comm(n, l.data(), buf, 0, nullptr);
+    if (per != 6) throw std::runtime_error("pack_comm size");
+    m->pair->unpack_comm(n, first, buf);
+  });
+}
+
+// fix adapt: pre_adapt("mu") then adapt(value); returns the index
+int mock_pair_adapt(mock_lmp* m, const char* name, double value) {
+  if (!m->pair) return -9;
+  std::string s(name);
+  const int w = m->pair->pre_adapt(s.data(), 1, 1, 1, 1);
+  if (w == 0) m->pair->adapt(w, 1, 1, 1, 1, value);
+  return w;
+}
+
+}  // extern "C\"

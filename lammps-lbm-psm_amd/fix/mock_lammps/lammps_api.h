@@ -6,6 +6,7 @@
 // sources compile inside a LAMMPS tree, where the real headers are used.
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -18,6 +19,19 @@
 #include <vector>
 
 #define FLERR __FILE__, __LINE__
+
+// single-process stand-in for the MPI calls the styles make
+typedef int MPI_Comm;
+#define MPI_INT 1
+#define MPI_DOUBLE 2
+inline int MPI_Bcast(void*, int, int, int, MPI_Comm) { return 0; }
+
+#ifndef MAX
+#define MAX(a, b) ((a) > (b) ? (a) : (b))
+#endif
+#ifndef MIN
+#define MIN(a, b) ((a) < (b) ? (a) : (b))
+#endif
 
 namespace LAMMPS_NS {
 
@@ -32,6 +46,7 @@ class Error {
   [[noreturn]] void all(const char* file, int line, const std::string& msg) {
     throw std::runtime_error(std::string(file) + ":" + std::to_string(line) + ": " + msg);
   }
+  [[noreturn]] void one(const char* file, int line, const std::string& msg) { all(file, line, msg); }
 };
 
 class Memory {
@@ -68,6 +83,11 @@ class Memory {
     return array;
   }
   template <typename T>
+  T** create(T**& array, int n1, int n2, const char* name) {
+    array = nullptr;
+    return grow(array, n1, n2, name);
+  }
+  template <typename T>
   void destroy(T**& array) {
     if (!array) return;
     auto it = blocks_.find((void*)array);
@@ -97,6 +117,9 @@ class Atom {
   double** omega = nullptr;
   double** torque = nullptr;
   double* radius = nullptr;
+  int* type = nullptr;
+  int ntypes = 1;
+  int omega_flag = 1, radius_flag = 1;
   // Atom::add_callback: the embedding program registers the fix that asked
   bool callback_requested = false;
   void add_callback(int) { callback_requested = true; }
@@ -104,6 +127,7 @@ class Atom {
 
 class Domain {
  public:
+  enum { NO_REMAP, X_REMAP, V_REMAP };
   int dimension = 3;
   int xperiodic = 1, yperiodic = 1, zperiodic = 1;
   double xprd = 1, yprd = 1, zprd = 1;
@@ -113,6 +137,7 @@ class Domain {
 class Comm {
  public:
   int me = 0, nprocs = 1;
+  int ghost_velocity = 1;
   int procgrid[3] = {1, 1, 1};
   int myloc[3] = {0, 0, 0};
   int procneigh[3][2] = {{0, 0}, {0, 0}, {0, 0}};
@@ -143,6 +168,32 @@ class Modify {
     fix = list.data();
     nfix = (int)list.size();
   }
+  // fixes whose style matches a pattern; only the "^name" prefix form
+  std::vector<Fix*> get_fix_by_style(const std::string& pattern) const;
+};
+
+class Force {
+ public:
+  int newton_pair = 0;
+  double vxmu2f = 1.0;  // units lj / si
+};
+
+namespace NeighConst {
+enum { REQ_DEFAULT = 0, REQ_FULL = 1 << 0 };
+}
+
+class NeighList {
+ public:
+  int inum = 0;
+  int* ilist = nullptr;
+  int* numneigh = nullptr;
+  int** firstneigh = nullptr;
+};
+
+class Neighbor {
+ public:
+  int requested = -1;  // flags of the last request
+  void add_request(void*, int flags = NeighConst::REQ_DEFAULT) { requested = flags; }
 };
 
 class LAMMPS {
@@ -154,8 +205,13 @@ class LAMMPS {
   Domain* domain = new Domain;
   Comm* comm = new Comm;
   Modify* modify = new Modify;
+  Force* force = new Force;
+  Neighbor* neighbor = new Neighbor;
+  MPI_Comm world = 0;
   std::string log;
   ~LAMMPS() {
+    delete force;
+    delete neighbor;
     delete memory;
     delete error;
     delete atom;
@@ -170,7 +226,8 @@ class Pointers {
  public:
   explicit Pointers(LAMMPS* ptr)
       : lmp(ptr), memory(ptr->memory), error(ptr->error), atom(ptr->atom),
-        update(ptr->update), domain(ptr->domain), comm(ptr->comm), modify(ptr->modify) {}
+        update(ptr->update), domain(ptr->domain), comm(ptr->comm), modify(ptr->modify),
+        force(ptr->force), neighbor(ptr->neighbor), world(ptr->world) {}
   virtual ~Pointers() = default;
 
  protected:
@@ -182,6 +239,9 @@ class Pointers {
   Domain*& domain;
   Comm*& comm;
   Modify*& modify;
+  Force*& force;
+  Neighbor*& neighbor;
+  MPI_Comm& world;
 };
 
 namespace FixConst {
@@ -231,12 +291,124 @@ class Fix : protected Pointers {
   }
 };
 
+// Fix styles the lubrication pair looks for (fix deform / fix wall).
+class FixDeform : public Fix {
+ public:
+  int remapflag = Domain::V_REMAP;
+  using Fix::Fix;
+  int setmask() override { return 0; }
+};
+class FixWall : public Fix {
+ public:
+  int xflag = 0;
+  using Fix::Fix;
+  int setmask() override { return 0; }
+};
+
+inline std::vector<Fix*> Modify::get_fix_by_style(const std::string& pattern) const {
+  std::vector<Fix*> out;
+  const std::string pre = (!pattern.empty() && pattern[0] == '^') ? pattern.substr(1) : pattern;
+  for (Fix* f : list)
+    if (std::string(f->style).compare(0, pre.size(), pre) == 0) out.push_back(f);
+  return out;
+}
+
+// Pair base class: the members and helpers a pair style uses.  ev_tally_xyz
+// accumulates the global and per-atom virial as LAMMPS does: with
+// newton_pair off each local atom of a pair contributes half of the pair's
+// virial.
+class Pair : protected Pointers {
+ public:
+  enum { GEOMETRIC, ARITHMETIC, SIXTHPOWER };
+  int allocated = 0;
+  int** setflag = nullptr;
+  double** cutsq = nullptr;
+  int single_enable = 1, comm_forward = 0, no_virial_fdotr_compute = 0;
+  int offset_flag = 0, mix_flag = GEOMETRIC;
+  int evflag = 0, eflag_global = 0, vflag_global = 0, vflag_atom = 0, vflag_fdotr = 0;
+  double eng_vdwl = 0.0, eng_coul = 0.0;
+  double virial[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<std::vector<double>> vatom;
+  NeighList* list = nullptr;
+
+  explicit Pair(LAMMPS* lmp) : Pointers(lmp) {}
+  ~Pair() override = default;
+  virtual void compute(int, int) = 0;
+  virtual void settings(int, char**) = 0;
+  virtual void coeff(int, char**) = 0;
+  virtual double init_one(int, int) = 0;
+  virtual void init_style() {}
+  virtual void write_restart(FILE*) {}
+  virtual void read_restart(FILE*) {}
+
+  double mix_distance(double a, double b) const {
+    if (mix_flag == GEOMETRIC) return std::sqrt(a * b);
+    if (mix_flag == ARITHMETIC) return 0.5 * (a + b);
+    return std::pow(0.5 * (std::pow(a, 6.0) + std::pow(b, 6.0)), 1.0 / 6.0);
+  }
+  void ev_setup(int eflag, int vflag) {
+    evflag = 1;
+    eflag_global = eflag ? 1 : 0;
+    vflag_global = (vflag & 3) ? 1 : 0;
+    vflag_atom = (vflag & 4) ? 1 : 0;
+    eng_vdwl = eng_coul = 0.0;
+    for (double& c : virial) c = 0.0;
+    if (vflag_atom) vatom.assign(atom->nmax, std::vector<double>(6, 0.0));
+  }
+  void ev_tally_xyz(int i, int j, int nlocal, int newton_pair, double, double, double fx,
+                    double fy, double fz, double delx, double dely, double delz) {
+    const double v[6] = {delx * fx, dely * fy, delz * fz, delx * fy, delx * fz, dely * fz};
+    auto add = [&](double* acc, double w) {
+      for (int c = 0; c < 6; ++c) acc[c] += w * v[c];
+    };
+    if (vflag_global) {
+      if (newton_pair) {
+        add(virial, 1.0);
+      } else {
+        if (i < nlocal) add(virial, 0.5);
+        if (j < nlocal) add(virial, 0.5);
+      }
+    }
+    if (vflag_atom) {
+      if ((newton_pair || i < nlocal) && i < (int)vatom.size()) add(vatom[i].data(), 0.5);
+      if ((newton_pair || j < nlocal) && j < (int)vatom.size()) add(vatom[j].data(), 0.5);
+    }
+  }
+};
+
+namespace MathConst {
+static constexpr double MY_PI = 3.14159265358979323846;
+}
+
 namespace utils {
 inline double numeric(const char* file, int line, const char* str, bool, LAMMPS* lmp) {
   char* end = nullptr;
   double v = std::strtod(str, &end);
   if (!end || *end != '\0') lmp->error->all(file, line, std::string("Expected floating point parameter instead of ") + str);
   return v;
+}
+inline int inumeric(const char* file, int line, const char* str, bool, LAMMPS* lmp) {
+  char* end = nullptr;
+  long v = std::strtol(str, &end, 10);
+  if (!end || *end != '\0') lmp->error->all(file, line, std::string("Expected integer parameter instead of ") + str);
+  return (int)v;
+}
+// "n", "*", "n*", "*n", "m*n" type ranges within [nmin, nmax]
+inline void bounds(const char* file, int line, const std::string& str, int nmin, int nmax,
+                   int& nlo, int& nhi, Error* error) {
+  const size_t star = str.find('*');
+  if (star == std::string::npos) {
+    nlo = nhi = std::atoi(str.c_str());
+  } else {
+    nlo = star == 0 ? nmin : std::atoi(str.substr(0, star).c_str());
+    nhi = star + 1 == str.size() ? nmax : std::atoi(str.substr(star + 1).c_str());
+  }
+  if (nlo < nmin || nhi > nmax || nlo > nhi)
+    error->all(file, line, "Numeric index " + str + " is out of bounds");
+}
+inline void sfread(const char* file, int line, void* ptr, size_t size, size_t num, FILE* fp,
+                   const char*, Error* error) {
+  if (std::fread(ptr, size, num, fp) != num) error->all(file, line, "Unexpected end of restart file");
 }
 inline void logmesg(LAMMPS* lmp, const std::string& s) { lmp->log += s; }
 }  // namespace utils

@@ -44,6 +44,23 @@ def _lib():
     lib.mock_init.argtypes = [C.c_void_p]
     lib.mock_step.restype = C.c_int
     lib.mock_step.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, dp, dp, dp, dp, ip, dp, dp, dp]
+    lib.mock_pair_env.restype = C.c_int
+    lib.mock_pair_env.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_char_p]
+    lib.mock_pair_style.restype = C.c_int
+    lib.mock_pair_style.argtypes = [C.c_void_p, C.c_char_p]
+    lib.mock_pair_coeff.restype = C.c_int
+    lib.mock_pair_coeff.argtypes = [C.c_void_p, C.c_char_p]
+    lib.mock_pair_compute.restype = C.c_int
+    lib.mock_pair_compute.argtypes = [C.c_void_p, C.c_int, C.c_int, ip, dp, dp, dp, dp, ip, ip,
+                                      C.c_int, dp, dp, dp]
+    lib.mock_pair_request.restype = C.c_int
+    lib.mock_pair_request.argtypes = [C.c_void_p]
+    lib.mock_pair_restart.restype = C.c_int
+    lib.mock_pair_restart.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+    lib.mock_pair_comm.restype = C.c_int
+    lib.mock_pair_comm.argtypes = [C.c_void_p, C.c_int, ip, C.c_int, dp]
+    lib.mock_pair_adapt.restype = C.c_int
+    lib.mock_pair_adapt.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
     return lib
 
 
@@ -106,3 +123,50 @@ class Lammps:
         self._ok(self._lib.mock_step(self._h, nl, n - nl, _p(tag, C.c_int), _p(x), _p(v), _p(w),
                                      _p(r), _p(owner, C.c_int), _p(f), _p(t), _p(vir)))
         return f, t, vir
+
+
+    # ---- pair_style lubricate/GRM/LBDEM -------------------------------------
+    def pair_env(self, ntypes=1, newton_pair=0, ghost_velocity=1, vxmu2f=1.0, extra_fix=""):
+        """Atom types, newton_pair, comm_modify vel, units factor and an
+        optional `deform v|x` / `wall 0|1` fix for init_style to find."""
+        self._ok(self._lib.mock_pair_env(self._h, ntypes, newton_pair, ghost_velocity, vxmu2f,
+                                         extra_fix.encode()))
+
+    def pair_style(self, line):
+        """`pair_style ...` without the word `pair_style`."""
+        self._ok(self._lib.mock_pair_style(self._h, line.encode()))
+
+    def pair_coeff(self, line):
+        self._ok(self._lib.mock_pair_coeff(self._h, line.encode()))
+
+    def pair_compute(self, x, v, omega, radius, types, nlocal, neigh, vflag=1):
+        """Forces, torques of the locals and the global virial on the full
+        neighbour list neigh (list of partner lists of the locals)."""
+        n = len(radius)
+        x, v, w, r = _d(x), _d(v), _d(omega), _d(radius)
+        ty = np.ascontiguousarray(types, dtype=np.int32)
+        num = np.ascontiguousarray([len(neigh[i]) for i in range(nlocal)], dtype=np.int32)
+        flat = np.ascontiguousarray([j for i in range(nlocal) for j in neigh[i]] or [0],
+                                    dtype=np.int32)
+        f = np.zeros((max(nlocal, 1), 3))
+        t = np.zeros((max(nlocal, 1), 3))
+        vir = np.zeros(6)
+        self._ok(self._lib.mock_pair_compute(self._h, nlocal, n - nlocal, _p(ty, C.c_int), _p(x),
+                                             _p(v), _p(w), _p(r), _p(num, C.c_int),
+                                             _p(flat, C.c_int), vflag, _p(f), _p(t), _p(vir)))
+        return f[:nlocal], t[:nlocal], vir
+
+    def pair_request(self):
+        return self._lib.mock_pair_request(self._h)
+
+    def pair_restart(self, path, write):
+        self._ok(self._lib.mock_pair_restart(self._h, str(path).encode(), 1 if write else 0))
+
+    def pair_comm(self, idx, first):
+        buf = np.zeros(6 * max(len(idx), 1))
+        l = np.ascontiguousarray(idx, dtype=np.int32)
+        self._ok(self._lib.mock_pair_comm(self._h, len(idx), _p(l, C.c_int), first, _p(buf)))
+        return buf[:6 * len(idx)]
+
+    def pair_adapt(self, name, value):
+        return self._lib.mock_pair_adapt(self._h, name.encode(), value)
