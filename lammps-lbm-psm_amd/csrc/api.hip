@@ -415,6 +415,7 @@ int run_map(lbmpsm_ctx* c) {
   pa.dt_d = c->cfg.dt_d;
   pa.count = B.count;
   pa.list = B.list;
+  pa.slow_count = B.slow_count;
   pa.bi = B.bi;
   pa.bj = B.bj;
   pa.bk = B.bk;
@@ -446,6 +447,8 @@ int run_map(lbmpsm_ctx* c) {
   ma.sfcode = c->sfcode;
   ma.zone_cap = c->zone_cap;
   ma.zone_span = c->zone_span;
+  ma.slow_list = B.slow_list;
+  ma.slow_count = B.slow_count;
   {
     ProfScope ps(c, "map_fraction");
     HIPCHK(c, launch_entry_sf(ma, c->stream));
@@ -620,6 +623,8 @@ int lbmpsm_create(const lbmpsm_config* cfg, lbmpsm_ctx** out) {
   B.nbricks = (int64_t)B.nbi * B.nbj * B.nbk;
   rc |= dev_alloc(c, &B.count, B.nbricks);
   rc |= dev_alloc(c, &B.list, (size_t)B.nbricks * kBrickCap);
+  rc |= dev_alloc(c, &B.slow_list, (size_t)B.nbricks);
+  rc |= dev_alloc(c, &B.slow_count, (size_t)1);
   if (rc) {
     std::string m = c->err;
     lbmpsm_destroy(c);
@@ -662,7 +667,7 @@ int lbmpsm_destroy(lbmpsm_ctx* c) {
   void* pp[] = {P.raw, P.xl, P.kin, P.zone, P.rzone, P.out, P.hpart};
   for (void* p : pp) dev_free(p);
   BrickDev& B = c->br;
-  void* bb[] = {B.count, B.list};
+  void* bb[] = {B.count, B.list, B.slow_list, B.slow_count};
   for (void* p : bb) dev_free(p);
   dev_free(c->d_scratch);
   dev_free(c->sfcode);

@@ -91,6 +91,8 @@ struct BrickDev {
   int64_t nbricks = 0;
   int* count = nullptr;     // [nbricks]
   int* list = nullptr;      // [nbricks][kBrickCap]
+  int* slow_list = nullptr; // [nbricks] bricks for the general map kernel
+  int* slow_count = nullptr;  // [1]
 };
 
 struct SlotsDev {
@@ -120,6 +122,8 @@ struct MapArgs {
   unsigned short* sfcode;  // [n][zone_cap] codes of each entry's zone, k fastest
   int zone_cap;            // codes per entry
   int zone_span;           // bound of every zone's extent per axis
+  int* slow_list;          // [nbricks] bricks k_map leaves to k_map_general
+  int* slow_count;         // their number (zeroed by k_prep_bin)
   int bi, bj, bk, nbi, nbj, nbk;
   double hsq_half;
 };
@@ -163,6 +167,7 @@ struct PrepArgs {
   double dx, Uc, u_lb, tc, dt_d;
   int* count;
   int* list;
+  int* slow_count;  // work-list length of k_map_general, zeroed here
   int bi, bj, bk, nbi, nbj, nbk;
 };
 hipError_t launch_fill_int(int* p, int64_t n, int value, hipStream_t s);

@@ -100,6 +100,7 @@ __device__ __forceinline__ void entry_geometry(const PrepArgs& a, int p, double 
 // stores it, and the lanes share the appends to the touched bricks.
 __global__ __launch_bounds__(64) void k_prep_bin(const PrepArgs a) {
   const int p = blockIdx.x;
+  if (p == 0 && threadIdx.x == 0) *a.slow_count = 0;  // the map kernels of this step fill it
   if (p >= a.n) return;
   double loc[3], r_lb;
   int z[6], rz[6];
@@ -149,7 +150,6 @@ struct MapEntry {
 struct NodeSlots {
   int t0, t1;
   double sf0, sf1;
-  double u0[3], u1[3];
   bool d0, d1;      // slot written this step
   bool k0, k1;      // sf0 / sf1 known (loaded or written)
 };
@@ -197,20 +197,33 @@ __device__ __forceinline__ void map_transition(const MapArgs& a, const MapEntry&
     else if (S.t1 == tag) sl = 1;
     else if (S.t0 == 0) sl = 0;
     else if (S.t1 == 0) sl = 1;
+    // the node velocity is stored at once (a later entry of the node stores
+    // after it); tag, solid fraction and force are stored at the end
+    double* us = a.s.us + (sl == 1 ? 3 * plane : 0) + n;
     if (sl == 0) {
       S.t0 = tag; S.sf0 = sf; S.k0 = true; S.d0 = true;
-      S.u0[0] = un[0]; S.u0[1] = un[1]; S.u0[2] = un[2];
     } else if (sl == 1) {
       S.t1 = tag; S.sf1 = sf; S.k1 = true; S.d1 = true;
-      S.u1[0] = un[0]; S.u1[1] = un[1]; S.u1[2] = un[2];
+    }
+    if (sl >= 0) {
+      us[0] = un[0];
+      us[plane] = un[1];
+      us[2 * plane] = un[2];
     }
   } else if (do_clear) {  // setToZero, src/LBM_PSM_lattice.cpp:302-324
+    int sl = -1;
     if (S.t0 == tag) {
       S.t0 = 0; S.sf0 = 0.0; S.k0 = true; S.d0 = true;
-      S.u0[0] = S.u0[1] = S.u0[2] = 0.0;
+      sl = 0;
     } else if (S.t1 == tag) {
       S.t1 = 0; S.sf1 = 0.0; S.k1 = true; S.d1 = true;
-      S.u1[0] = S.u1[1] = S.u1[2] = 0.0;
+      sl = 1;
+    }
+    if (sl >= 0) {
+      double* us = a.s.us + 3 * plane * sl + n;
+      us[0] = 0.0;
+      us[plane] = 0.0;
+      us[2 * plane] = 0.0;
     }
   }
 }
@@ -519,7 +532,7 @@ __device__ __forceinline__ double map_code_sf(int dim, unsigned short cd) {
 #endif
 constexpr int kMapBatch = LBMPSM_MAP_BATCH;
 #ifndef LBMPSM_MAP_WAVES
-#define LBMPSM_MAP_WAVES 0  // > 0: ask for this many waves per SIMD for k_map
+#define LBMPSM_MAP_WAVES 8  // > 0: ask for this many waves per SIMD for k_map
 #endif
 #if LBMPSM_MAP_WAVES > 0
 #define LBMPSM_MAP_OCC __attribute__((amdgpu_waves_per_eu(LBMPSM_MAP_WAVES)))
@@ -527,16 +540,18 @@ constexpr int kMapBatch = LBMPSM_MAP_BATCH;
 #define LBMPSM_MAP_OCC
 #endif
 
-__global__ __launch_bounds__(256) LBMPSM_MAP_OCC void k_map(const MapArgs a) {
-  __shared__ MapEntry ent[kBrickCap];
-  __shared__ int raw[kBrickCap];
-  __shared__ int perm[kBrickCap];   // list order -> staging slot
-  const int64_t b = blockIdx.x;
+template <bool GENERAL>
+__device__ __forceinline__ void map_brick(const MapArgs& a, int64_t b, MapEntry* ent, int* raw,
+                                          int* perm, int* flag) {
   const int tid = threadIdx.x;
   // independent loads first: the brick's count and list, the node's slots
   const int m = a.count[b];
   const int rv = tid < kBrickCap ? a.list[b * kBrickCap + tid] : 0;
   if (m == 0) return;
+  if (!GENERAL && m > kBrickCap) {  // longer than its list: the general kernel
+    if (tid == 0) a.slow_list[atomicAdd(a.slow_count, 1)] = (int)b;
+    return;
+  }
   const Geom& g = a.g;
   const int64_t plane = g.plane;
   const bool listed = m <= kBrickCap;
@@ -555,7 +570,6 @@ __global__ __launch_bounds__(256) LBMPSM_MAP_OCC void k_map(const MapArgs a) {
   S.t0 = valid ? a.s.tag[n] : 0;
   S.t1 = valid ? a.s.tag[plane + n] : 0;
   S.d0 = S.d1 = false;
-  for (int c = 0; c < 3; ++c) S.u0[c] = S.u1[c] = 0.0;
 #if LBMPSM_MAP_EAGER
   // solid fractions of occupied slots are needed whenever an entry of the
   // same tag arrives; load them now, alongside the list
@@ -580,13 +594,23 @@ __global__ __launch_bounds__(256) LBMPSM_MAP_OCC void k_map(const MapArgs a) {
       for (int c = 0; c < 4; ++c) E.xl[c] = a.xl[4 * rv + c];
       for (int c = 0; c < 6; ++c) E.kin[c] = a.kin[6 * rv + c];
     }
+    if (!GENERAL && tid == 0) *flag = 0;
     __syncthreads();
     if (tid < m) {
       int r = 0;
       for (int u = 0; u < m; ++u) r += raw[u] < rv;
       perm[r] = tid;
+      if (!GENERAL) {
+        const int* z = ent[tid].zone;
+        if (!zone_coded(a, z[1] - z[0], z[3] - z[2], z[5] - z[4])) *flag = 1;
+      }
     }
     __syncthreads();
+    // an entry without codes: the general kernel takes the whole brick
+    if (!GENERAL && *flag) {
+      if (tid == 0) a.slow_list[atomicAdd(a.slow_count, 1)] = (int)b;
+      return;
+    }
     if (valid) {
       for (int r0 = 0; r0 < m; r0 += kMapBatch) {
         unsigned short cd[kMapBatch];
@@ -612,16 +636,17 @@ __global__ __launch_bounds__(256) LBMPSM_MAP_OCC void k_map(const MapArgs a) {
         for (int u = 0; u < kMapBatch; ++u) {
           if (!(in & (1u << u))) continue;
           const MapEntry& E = ent[perm[r0 + u]];
-          const double sf = (coded & (1u << u))
-                                ? map_code_sf(g.dim, cd[u])
-                                : solid_fraction(g.dim, a.hsq_half, ri, rj, rk, E.xl[0], E.xl[1],
-                                                 E.xl[2], E.xl[3]);
+          double sf;
+          if (!GENERAL || (coded & (1u << u)))
+            sf = map_code_sf(g.dim, cd[u]);
+          else
+            sf = solid_fraction(g.dim, a.hsq_half, ri, rj, rk, E.xl[0], E.xl[1], E.xl[2], E.xl[3]);
           map_transition(a, E, sf, ri, rj, rk, n, S);
         }
       }
     }
     __syncthreads();
-  } else {
+  } else if constexpr (GENERAL) {
     // more entries touch the brick than its list holds: replay the whole
     // particle list, staged kBrickCap entries at a time
     for (int base = 0; base < a.n; base += kBrickCap) {
@@ -661,9 +686,6 @@ __global__ __launch_bounds__(256) LBMPSM_MAP_OCC void k_map(const MapArgs a) {
   if (S.d0) {
     a.s.tag[n] = S.t0;
     a.s.sf[n] = S.sf0;
-    a.s.us[n] = S.u0[0];
-    a.s.us[plane + n] = S.u0[1];
-    a.s.us[2 * plane + n] = S.u0[2];
     a.s.Fh[n] = 0.0;
     a.s.Fh[plane + n] = 0.0;
     a.s.Fh[2 * plane + n] = 0.0;
@@ -671,14 +693,36 @@ __global__ __launch_bounds__(256) LBMPSM_MAP_OCC void k_map(const MapArgs a) {
   if (S.d1) {
     a.s.tag[plane + n] = S.t1;
     a.s.sf[plane + n] = S.sf1;
-    a.s.us[3 * plane + n] = S.u1[0];
-    a.s.us[4 * plane + n] = S.u1[1];
-    a.s.us[5 * plane + n] = S.u1[2];
     a.s.Fh[3 * plane + n] = 0.0;
     a.s.Fh[4 * plane + n] = 0.0;
     a.s.Fh[5 * plane + n] = 0.0;
   }
   if (S.d0 || S.d1) a.s.mask[n] = (uint8_t)((S.t0 > 0 ? 1 : 0) | (S.t1 > 0 ? 2 : 0));
+}
+
+// The common case: bricks whose entries fit their list and all have codes.
+// Other bricks go, untouched, to a work list for k_map_general.
+__global__ __launch_bounds__(256) LBMPSM_MAP_OCC void k_map(const MapArgs a) {
+  __shared__ MapEntry ent[kBrickCap];
+  __shared__ int raw[kBrickCap];
+  __shared__ int perm[kBrickCap];   // list order -> staging slot
+  __shared__ int flag;
+  map_brick<false>(a, blockIdx.x, ent, raw, perm, &flag);
+}
+
+// The bricks k_map passed on: overflowing lists and entries whose zones
+// have no codes (solid fractions computed here, cell by cell).  A fixed grid
+// walks the work list; an empty list costs one load per workgroup.
+constexpr int kMapGeneralBlocks = 256;
+__global__ __launch_bounds__(256) void k_map_general(const MapArgs a) {
+  __shared__ MapEntry ent[kBrickCap];
+  __shared__ int raw[kBrickCap];
+  __shared__ int perm[kBrickCap];
+  const int cnt = *a.slow_count;
+  for (int w = blockIdx.x; w < cnt; w += gridDim.x) {
+    map_brick<true>(a, a.slow_list[w], ent, raw, perm, nullptr);
+    __syncthreads();
+  }
 }
 
 // Force, torque and stresslet of one list entry, split over kHydroGroups
@@ -909,8 +953,12 @@ hipError_t launch_entry_sf(const MapArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_map(const MapArgs& a, int64_t nbricks, hipStream_t s) {
-  if (nbricks <= 0) return hipSuccess;
+  // without entries no node changes state (and k_prep_bin, which zeroes the
+  // general kernel's work list, did not run)
+  if (nbricks <= 0 || a.n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_map, dim3((unsigned)nbricks), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_map_general, dim3((unsigned)std::min<int64_t>(nbricks, kMapGeneralBlocks)),
+                     dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

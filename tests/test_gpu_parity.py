@@ -131,6 +131,25 @@ def test_three_particles_one_node_drop():
     compare(lat, ora, out)
 
 
+def test_crowded_bricks_overflow_their_lists():
+    # 7x7x9 tiny spheres 1.2 apart: more entries touch one 4x4x16-node brick
+    # than its list holds, so the map replays the whole particle list there
+    n = (24, 24, 32)
+    per = (1, 1, 1)
+    g = np.arange(7) * 1.2 + 6.3
+    gz = np.arange(9) * 1.2 + 9.1
+    pos = np.array([[x, y, z] for x in g for y in g for z in gz])
+    box = lattice_box(3, n, per)
+    atoms = random_atoms(box, [0.45] * len(pos), seed=9, positions=pos, vmax=2e-2, wmax=1e-2)
+    # entries whose (narrowed) zone reaches the brick around (8, 8, 16)
+    lo = np.array([8, 8, 0]) // [4, 4, 16] * [4, 4, 16]
+    near = np.all((pos + 1.45 >= lo) & (pos - 1.45 < lo + [4, 4, 16]), axis=1)
+    assert near.sum() > 64
+    lat, ora, out = run_pair(3, n, per, 0.8, (1e-6, 0.0, 0.0), "none", atoms, 3,
+                             cutoff=2.0, label="crowded")
+    compare(lat, ora, out)
+
+
 @pytest.mark.parametrize("bc", ["shear", "xFlow", "channel", "channel-velIn-pressOut",
                                 "closedBox"])
 @pytest.mark.parametrize("per", [(0, 0, 0), (1, 0, 1), (1, 1, 1), (0, 1, 1)])
