@@ -353,21 +353,27 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
       double B2 = o2 ? a.sf[plane + n] : 0.0;
       double us1[3] = {0.0, 0.0, 0.0}, us2[3] = {0.0, 0.0, 0.0};
       double Fa[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      // node forces continue from the stored value, or from zero for a slot
+      // the map set this step
       if (o1) {
         us1[0] = a.us[n];
         us1[1] = a.us[plane + n];
         us1[2] = a.us[2 * plane + n];
-        Fa[0] = a.Fh[n];
-        Fa[1] = a.Fh[plane + n];
-        Fa[2] = a.Fh[2 * plane + n];
+        if (!(m & kMaskFresh0)) {
+          Fa[0] = a.Fh[n];
+          Fa[1] = a.Fh[plane + n];
+          Fa[2] = a.Fh[2 * plane + n];
+        }
       }
       if (o2) {
         us2[0] = a.us[3 * plane + n];
         us2[1] = a.us[4 * plane + n];
         us2[2] = a.us[5 * plane + n];
-        Fa[3] = a.Fh[3 * plane + n];
-        Fa[4] = a.Fh[4 * plane + n];
-        Fa[5] = a.Fh[5 * plane + n];
+        if (!(m & kMaskFresh1)) {
+          Fa[3] = a.Fh[3 * plane + n];
+          Fa[4] = a.Fh[4 * plane + n];
+          Fa[5] = a.Fh[5 * plane + n];
+        }
       }
       const double u1sq = 0.5 * (us1[0] * us1[0] + us1[1] * us1[1] + us1[2] * us1[2]) * inv2;
       const double u2sq = 0.5 * (us2[0] * us2[0] + us2[1] * us2[1] + us2[2] * us2[2]) * inv2;
@@ -435,6 +441,7 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
         a.Fh[4 * plane + n] = Fa[4];
         a.Fh[5 * plane + n] = Fa[5];
       }
+      if (m & (kMaskFresh0 | kMaskFresh1)) a.mask[n] = m & (kMaskOcc0 | kMaskOcc1);
     }
   }
 }
@@ -604,16 +611,21 @@ __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, uns
         if constexpr (cy != 0) { Fa[1] += cy * t1; Fa[4] += cy * t2; }
         if constexpr (cz != 0) { Fa[2] += cz * t1; Fa[5] += cz * t2; }
       });
+      // node forces continue from the stored value, or from zero for a slot
+      // the map set this step
       if (o1) {
-        a.Fh[n] = a.Fh[n] + (double)Fa[0];
-        a.Fh[plane + n] = a.Fh[plane + n] + (double)Fa[1];
-        a.Fh[2 * plane + n] = a.Fh[2 * plane + n] + (double)Fa[2];
+        const bool fr = m & kMaskFresh0;
+        a.Fh[n] = (fr ? 0.0 : a.Fh[n]) + (double)Fa[0];
+        a.Fh[plane + n] = (fr ? 0.0 : a.Fh[plane + n]) + (double)Fa[1];
+        a.Fh[2 * plane + n] = (fr ? 0.0 : a.Fh[2 * plane + n]) + (double)Fa[2];
       }
       if (o2) {
-        a.Fh[3 * plane + n] = a.Fh[3 * plane + n] + (double)Fa[3];
-        a.Fh[4 * plane + n] = a.Fh[4 * plane + n] + (double)Fa[4];
-        a.Fh[5 * plane + n] = a.Fh[5 * plane + n] + (double)Fa[5];
+        const bool fr = m & kMaskFresh1;
+        a.Fh[3 * plane + n] = (fr ? 0.0 : a.Fh[3 * plane + n]) + (double)Fa[3];
+        a.Fh[4 * plane + n] = (fr ? 0.0 : a.Fh[4 * plane + n]) + (double)Fa[4];
+        a.Fh[5 * plane + n] = (fr ? 0.0 : a.Fh[5 * plane + n]) + (double)Fa[5];
       }
+      if (m & (kMaskFresh0 | kMaskFresh1)) a.mask[n] = m & (kMaskOcc0 | kMaskOcc1);
     }
   }
 }

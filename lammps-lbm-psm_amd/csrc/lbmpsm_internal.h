@@ -33,7 +33,8 @@ struct CollideArgs {
   Geom g;
   const void* fin;
   void* fout;
-  const uint8_t* mask;  // bit s set: slot s holds a particle (tag > 0)
+  uint8_t* mask;        // kMaskOcc*: slot holds a particle; kMaskFresh*: set
+                        // this step (force starts at zero; cleared here)
   const double* sf;     // [2][plane]
   const double* us;     // [6][plane]   (3*s + d)
   double* Fh;           // [6][plane]   (3*s + d), lattice units
@@ -59,6 +60,11 @@ struct CollideArgs {
 #define LBMPSM_HYDRO_GROUPS 4
 #endif
 constexpr int kHydroGroups = LBMPSM_HYDRO_GROUPS;
+
+// Per-node slot mask bits: slot s holds a particle (tag > 0); slot s was set
+// by this step's map, so its node force starts from zero in the collide
+// (which then clears the bit).
+constexpr uint8_t kMaskOcc0 = 1, kMaskOcc1 = 2, kMaskFresh0 = 4, kMaskFresh1 = 8;
 
 // Particle list state on the device.  The raw LAMMPS arrays of one call
 // arrive in one copy: x[n][3], v[n][3], omega[n][3], radius[n] (doubles)

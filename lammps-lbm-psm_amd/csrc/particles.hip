@@ -683,21 +683,32 @@ __device__ __forceinline__ void map_brick(const MapArgs& a, int64_t b, MapEntry*
   // the brick list is consumed: leave its count at zero for the next map
   if (tid == 0) a.count_reset[b] = 0;
   if (!valid) return;
+  // a slot cleared this step (setToZero) keeps no force
   if (S.d0) {
     a.s.tag[n] = S.t0;
     a.s.sf[n] = S.sf0;
-    a.s.Fh[n] = 0.0;
-    a.s.Fh[plane + n] = 0.0;
-    a.s.Fh[2 * plane + n] = 0.0;
+    if (S.t0 <= 0) {
+      a.s.Fh[n] = 0.0;
+      a.s.Fh[plane + n] = 0.0;
+      a.s.Fh[2 * plane + n] = 0.0;
+    }
   }
   if (S.d1) {
     a.s.tag[plane + n] = S.t1;
     a.s.sf[plane + n] = S.sf1;
-    a.s.Fh[3 * plane + n] = 0.0;
-    a.s.Fh[4 * plane + n] = 0.0;
-    a.s.Fh[5 * plane + n] = 0.0;
+    if (S.t1 <= 0) {
+      a.s.Fh[3 * plane + n] = 0.0;
+      a.s.Fh[4 * plane + n] = 0.0;
+      a.s.Fh[5 * plane + n] = 0.0;
+    }
   }
-  if (S.d0 || S.d1) a.s.mask[n] = (uint8_t)((S.t0 > 0 ? 1 : 0) | (S.t1 > 0 ? 2 : 0));
+  // a slot set this step has its force reset (src/LBM_PSM_lattice.cpp:
+  // 257-299): mark it fresh (mask bits 2, 3) and the collide starts its
+  // force from zero instead of reading it
+  if (S.d0 || S.d1)
+    a.s.mask[n] = (uint8_t)((S.t0 > 0 ? kMaskOcc0 : 0) | (S.t1 > 0 ? kMaskOcc1 : 0) |
+                            (S.d0 && S.t0 > 0 ? kMaskFresh0 : 0) |
+                            (S.d1 && S.t1 > 0 ? kMaskFresh1 : 0));
 }
 
 // The common case: bricks whose entries fit their list and all have codes.
