@@ -4,6 +4,13 @@
   centre, body force Fx = 1e-6, rest start, 5000 steps: the library against
   the CPU oracle, drag within 1e-12 of the sum of its terms' magnitudes and
   the velocity field within 1e-12 relative.
+* D3Q19 fluidised-bed column 1024 x 256 x 256 (x vertical), xFlow: Zou-He
+  velocity inlet U = 0.005 at x = 0 and density outlet, periodic y/z,
+  gravity Fx = -2e-6, 20000 fixed spheres d = 12, tau 0.7, FP64 and FP32
+  storage: FP32 drag/torque within 1e-5 of FP64 after 300 steps, and the
+  step rate of both forms.  The spheres are seeded over the whole column
+  (phi = 0.27): 20000 of them in its lower third would need phi = 0.81,
+  beyond random sequential addition.
 * D3Q19 256^3 periodic, Fx = 5e-6, 30 % volume fraction of fixed spheres with
   d ~ U[10, 20] (seeded RSA, largest first), tau in {0.6, 0.65, 0.8}, 3000
   steps in FP64, FP32 and FP16 storage: per-sphere drag, torque and
@@ -95,3 +102,86 @@ def test_dense_polydisperse_reduced_storage(tau):
         eS = np.max(np.abs(S - S64)) / np.max(np.abs(S64))
         print(f"  {storage} vs f64: drag {eF:.2e} torque {eT:.2e} stresslet {eS:.2e}", flush=True)
         assert eF <= bound and eT <= bound and eS <= bound
+
+
+def _rsa_grid(L, r, n, seed, lo_x, hi_x):
+    """Random sequential addition of n equal spheres (radius r) in a box
+    periodic in y and z, centres in [lo_x, hi_x) along x, with a cell grid."""
+    rng = np.random.default_rng(seed)
+    cell = 2.0 * r
+    nc = [max(1, int(L[d] // cell)) for d in range(3)]
+    grid = {}
+    pos = []
+    tries = 0
+    while len(pos) < n:
+        tries += 1
+        assert tries < 50 * n, "RSA jammed"
+        c = np.array([rng.uniform(lo_x, hi_x), rng.uniform(0, L[1]), rng.uniform(0, L[2])])
+        ci = [int(c[d] // cell) % nc[d] for d in range(3)]
+        ok = True
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    key = ((ci[0] + dx) % nc[0], (ci[1] + dy) % nc[1], (ci[2] + dz) % nc[2])
+                    for q in grid.get(key, ()):
+                        d = c - pos[q]
+                        d[1] -= L[1] * round(d[1] / L[1])
+                        d[2] -= L[2] * round(d[2] / L[2])
+                        if d @ d < (2.0 * r) ** 2:
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if not ok:
+                break
+        if ok:
+            grid.setdefault(tuple(ci), []).append(len(pos))
+            pos.append(c)
+    return np.array(pos)
+
+
+@pytest.mark.slow
+def test_fluidised_bed_xflow_fp32_vs_fp64():
+    import time
+    n = (1024, 256, 256)
+    per = (0, 1, 1)
+    box = lattice_box(3, n, per)
+    L = box.prd
+    pos = _rsa_grid(L, 6.0, 20000, 505, 20.0, L[0] - 20.0)
+    atoms = H.Atoms(tag=np.arange(1, len(pos) + 1, dtype=np.int32), x=pos,
+                    v=np.zeros((len(pos), 3)), omega=np.zeros((len(pos), 3)),
+                    radius=np.full(len(pos), 6.0))
+    lst = H.build_list(atoms, box, 10.0)
+    print(f"\n[case] D3Q19 1024x256x256 xFlow U=0.005, Fx=-2e-6, {atoms.n} spheres d=12 "
+          f"({lst.n} entries), tau 0.7, 300 steps x FP64/FP32", flush=True)
+    res = {}
+    for storage in ("f64", "f32"):
+        lat = lbmpsm.Lattice(lbmpsm.lattice_units_config(3, n, per, 0.7, F_lbm=(-2e-6, 0.0, 0.0),
+                                                         storage=storage, u_lb=0.005))
+        lat.set_bc("xFlow")
+        lat.set_particles(lst.tag, lst.x, lst.v, lst.omega, lst.radius)
+        for step in range(250):
+            lat.step()
+        lat.sync()
+        t0 = time.perf_counter()
+        for step in range(50):
+            lat.set_particles(lst.tag, lst.x, lst.v, lst.omega, lst.radius)
+            lat.step()
+        lat.sync()
+        dt = (time.perf_counter() - t0) / 50
+        F, T, S = lat.hydro()
+        Fo, To = H.reduce_to_owners(lst, F, T)
+        res[storage] = (Fo, To)
+        lat.close()
+        print(f"  {storage}: {n[0] * n[1] * n[2] / dt / 1e6:.0f} MLUPS ({dt * 1e3:.2f} ms/step, "
+              f"host list each step), mean drag {np.mean(Fo[:, 0]):.6e}", flush=True)
+        assert np.all(np.isfinite(Fo))
+    F64, T64 = res["f64"]
+    F32, T32 = res["f32"]
+    eF = np.max(np.linalg.norm(F32 - F64, axis=1)) / np.max(np.linalg.norm(F64, axis=1))
+    eT = np.max(np.linalg.norm(T32 - T64, axis=1)) / np.max(np.linalg.norm(T64, axis=1))
+    print(f"  f32 vs f64: drag {eF:.2e} torque {eT:.2e}", flush=True)
+    assert np.mean(F64[:, 0]) > 0.0   # the rising flow drags the bed upwards
+    assert eF <= 1e-5 and eT <= 1e-5
