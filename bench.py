@@ -180,8 +180,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the lattice would shard by slabs with a 5-population face exchange; "
-              "see DESIGN.md).", file=sys.stderr)
+              "(the lattice would shard by slabs along one axis, exchanging the 5 "
+              "populations that cross each face every step).", file=sys.stderr)
         sys.exit(2)
 
     box, atoms = workload()

@@ -16,6 +16,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "lbmpsm_internal.h"
 
@@ -492,6 +493,10 @@ __device__ __forceinline__ float eq_odd(float Ax, float Ay, float Az) {
   }
 }
 
+template <int DIM, int Q, typename Emit>
+__device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f, unsigned n,
+                                                  Emit&& emit);
+
 template <int DIM, int Q, int ST>
 __global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_lowp(const CollideArgs a) {
   const Geom& g = a.g;
@@ -510,10 +515,22 @@ __global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_lowp(const CollideAr
 template <int DIM, int Q, int ST>
 __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, unsigned rb,
                                              unsigned uc) {
+  using T = typename StoreT<ST>::T;
+  const int64_t plane = a.g.plane;
+  collide_lowp_node<DIM, Q>(a, f, rb + uc, [&](auto qc, float out) {
+    st<ST>(lane_at(pq_ptr_w<T>(a.fout, decltype(qc)::value, plane, rb), uc), out, a.scale);
+  });
+}
+
+// One node in FP32: f holds the node's post-stream populations (after the
+// faces); emit(q, value) receives each post-collision population in q order
+// and the node's force goes to Fh.
+template <int DIM, int Q, typename Emit>
+__device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f, unsigned n,
+                                                  Emit&& emit) {
 #pragma clang fp contract(fast)
   const Geom& g = a.g;
   const int64_t plane = g.plane;
-  using T = typename StoreT<ST>::T;
   const float omega = (float)a.omega;
   const float rho0 = (float)a.rho0;
   const float F0 = (float)a.F[0], F1 = (float)a.F[1], F2 = (float)a.F[2];
@@ -521,7 +538,6 @@ __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, uns
               hF2 = (float)(0.5 * OddK<Q>::k * a.F[2]);
   const float gp = (float)a.guo_pref;
   {
-    const unsigned n = rb + uc;
     float drho, jx, jy, jz;
     moments<Q, float>(f, drho, jx, jy, jz);
     const float rho = rho0 + drho;
@@ -553,7 +569,7 @@ __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, uns
           const float eF = edot<Q, q>(F0, F1, F2);
           out += gp * w * (3.0f * ((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) + 9.0f * eu * eF);
         }
-        st<ST>(lane_at(pq_ptr_w<T>(a.fout, q, plane, rb), uc), out, a.scale);
+        emit(std::integral_constant<int, q>{}, out);
       });
     } else {
       const bool o1 = m & 1, o2 = m & 2;
@@ -605,7 +621,7 @@ __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, uns
           const float eF = edot<Q, q>(F0, F1, F2);
           out += gp * w * (3.0f * ((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) + 9.0f * eu * eF);
         }
-        st<ST>(lane_at(pq_ptr_w<T>(a.fout, q, plane, rb), uc), out, a.scale);
+        emit(std::integral_constant<int, q>{}, out);
         const float t1 = -B1 * sc1, t2 = -B2 * sc2;
         if constexpr (cx != 0) { Fa[0] += cx * t1; Fa[3] += cx * t2; }
         if constexpr (cy != 0) { Fa[1] += cy * t1; Fa[4] += cy * t2; }
