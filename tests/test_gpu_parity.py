@@ -131,6 +131,42 @@ def test_three_particles_one_node_drop():
     compare(lat, ora, out)
 
 
+def test_particle_leaves_list_and_returns():
+    # a sphere missing from the list for some steps leaves stale slots: they
+    # keep their state and their node forces keep accumulating; when it comes
+    # back its slots are set again (force restarted from zero)
+    n = (24, 22, 20)
+    per = (1, 1, 1)
+    box = lattice_box(3, n, per)
+    atoms = random_atoms(box, [3.2, 2.8, 3.0], seed=17, vmax=8e-3, wmax=1e-3, gap=1.5)
+    lat = lbmpsm.Lattice(lbmpsm.lattice_units_config(3, n, per, 0.7, F_lbm=(1e-6, 0, 0)))
+    ora = O.Oracle(3, n, per, 0.7, F_lbm=(1e-6, 0, 0))
+    print("\n[case] particle leaves the list for steps 4-7 and returns", flush=True)
+    out = []
+    for step in range(12):
+        H.move_linear(atoms, box, step, 1.0)
+        if 4 <= step < 8:  # tag 2 (and its images) is missing from the list
+            k = atoms.tag != 2
+            sub = H.Atoms(tag=atoms.tag[k], x=atoms.x[k], v=atoms.v[k], omega=atoms.omega[k],
+                          radius=atoms.radius[k])
+            lst = H.build_list(sub, box, 7.5)
+        else:
+            lst = H.build_list(atoms, box, 7.5)
+        Fo, To, So = ora.fix_step(lst, bc_keyword=0)
+        lat.set_particles(lst.tag, lst.x, lst.v, lst.omega, lst.radius)
+        lat.step(store_macros=True)
+        Fg, Tg, Sg = lat.hydro()
+        out.append((lst, (Fg, Tg, Sg), (Fo, To, So)))
+        if step == 7:
+            tag, _, _, Fh = lat.slots()
+            s0, s1 = tag[..., 0] == 2, tag[..., 1] == 2
+            assert np.any(s0 | s1), "no stale slot of the missing sphere"
+            assert np.any(Fh[..., 0:3][s0] != 0.0) or np.any(Fh[..., 3:6][s1] != 0.0), \
+                "stale slots carry no force"
+            compare(lat, ora, out)
+    compare(lat, ora, out)
+
+
 def test_crowded_bricks_overflow_their_lists():
     # 7x7x9 tiny spheres 1.2 apart: more entries touch one 4x4x16-node brick
     # than its list holds, so the map replays the whole particle list there
