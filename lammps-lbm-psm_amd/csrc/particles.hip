@@ -737,22 +737,20 @@ __global__ __launch_bounds__(256) void k_map_general(const MapArgs a) {
 }
 
 // Force, torque and stresslet of one list entry, split over kHydroGroups
-// workgroups of kHydroThreads threads.  The entry's reduction zone (nodes
-// in i, j, k order) is cut into kHydroGroups equal ranges, one per
-// workgroup; each thread sums its nodes of the range, kHydroU at a time so
-// that their loads are in flight together.  The sums are combined in a
-// fixed order: a shuffle tree within each wave, the wave sums in wave order,
-// then (k_hydro_sum) the group sums in group order.  Terms as src/LBM_PSM_exchangeParticleData.cpp:197-240:
+// workgroups of kHydroThreads threads.  Group g takes the x-planes
+// [g ni / G, (g+1) ni / G) of the entry's reduction zone; a thread takes (y, z)
+// columns of the zone and, for each, loads the slot tags of all its planes
+// together (kHydroU at a time), then the node forces of the matching slots.
+// The sums are combined in a fixed order: a shuffle tree within each wave,
+// the wave sums in wave order, then (k_hydro_sum) the group sums in group
+// order.  Terms as src/LBM_PSM_exchangeParticleData.cpp:197-240:
 // F += Fhyd ff, T += (r x Fhyd) tf, S -= 1/2 (r_a F_b + r_b F_a) tf with r
 // the node offset from the entry's lattice position.
 #ifndef LBMPSM_HYDRO_THREADS
 #define LBMPSM_HYDRO_THREADS 256
 #endif
-#ifndef LBMPSM_HYDRO_GROUPS
-#define LBMPSM_HYDRO_GROUPS 4
-#endif
 #ifndef LBMPSM_HYDRO_U
-#define LBMPSM_HYDRO_U 4
+#define LBMPSM_HYDRO_U 3
 #endif
 constexpr int kHydroThreads = LBMPSM_HYDRO_THREADS;
 constexpr int kHydroU = LBMPSM_HYDRO_U;
@@ -771,55 +769,51 @@ __global__ __launch_bounds__(kHydroThreads) void k_hydro(const HydroArgs a) {
   double acc[12];
   for (int c = 0; c < 12; ++c) acc[c] = 0.0;
   const int ni = z[1] - z[0], nj = z[3] - z[2], nk = z[5] - z[4];
-  const int tot = (ni > 0 && nj > 0 && nk > 0) ? ni * nj * nk : 0;
-  const int njk = nj * nk;
-  const int chunk = (tot + kHydroGroups - 1) / kHydroGroups;
-  const int lo = grp * chunk, hi = min(tot, lo + chunk);
-  // 32-bit index arithmetic (a zone holds far fewer than 2^31 nodes)
+  const bool empty = ni <= 0 || nj <= 0 || nk <= 0;
+  const int plo = empty ? 0 : (grp * ni) / kHydroGroups;
+  const int phi = empty ? 0 : ((grp + 1) * ni) / kHydroGroups;
+  const int npairs = empty ? 0 : nj * nk;
+  const int psz = g.sy * g.sz;  // nodes per x-plane (32-bit: fewer than 2^31 stored nodes)
   constexpr int U = kHydroU;
-  for (int base = lo + (int)threadIdx.x; base < hi; base += kHydroThreads * U) {
-    int nn[U], sl[U], ii[U], jj[U], kk[U];
+  for (int pr = (int)threadIdx.x; pr < npairs; pr += kHydroThreads) {
+    const int lj = pr / nk, lk = pr - lj * nk;
+    const int j = z[2] + lj, k = z[4] + lk;
+    const int nb = j * g.sz + k;
+    const double dy = (double)(j + g.off[1]) - yL;
+    const double dz = (double)(k + g.off[2]) - zL;
+    for (int i0 = plo; i0 < phi; i0 += U) {
+      int sl[U], nn[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int t = base + kHydroThreads * u;
-      sl[u] = -1;
-      nn[u] = 0;
-      ii[u] = jj[u] = kk[u] = 0;
-      if (t < hi) {
-        const int li = t / njk;
-        const int rem = t - li * njk;
-        const int lj = rem / nk;
-        const int lk = rem - lj * nk;
-        ii[u] = z[0] + li;
-        jj[u] = z[2] + lj;
-        kk[u] = z[4] + lk;
-        nn[u] = (ii[u] * g.sy + jj[u]) * g.sz + kk[u];
-        const int a0 = a.slot_tag[nn[u]], a1 = a.slot_tag[plane + nn[u]];
-        sl[u] = tag == a0 ? 0 : (tag == a1 ? 1 : -1);
+      for (int u = 0; u < U; ++u) {
+        sl[u] = -1;
+        nn[u] = 0;
+        if (i0 + u < phi) {
+          nn[u] = (z[0] + i0 + u) * psz + nb;
+          const int a0 = a.slot_tag[nn[u]], a1 = a.slot_tag[plane + nn[u]];
+          sl[u] = tag == a0 ? 0 : (tag == a1 ? 1 : -1);
+        }
       }
-    }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (sl[u] < 0) continue;
-      const int s = sl[u], n = nn[u];
-      const double F0 = a.Fh[(3 * s) * plane + n];
-      const double F1 = a.Fh[(3 * s + 1) * plane + n];
-      const double F2 = a.Fh[(3 * s + 2) * plane + n];
-      acc[0] += F0 * ff;
-      acc[1] += F1 * ff;
-      acc[2] += F2 * ff;
-      const double dx = (double)(ii[u] + g.off[0]) - xL;
-      const double dy = (double)(jj[u] + g.off[1]) - yL;
-      const double dz = (double)(kk[u] + g.off[2]) - zL;
-      acc[3] += (dy * F2 - dz * F1) * tf;
-      acc[4] += (dz * F0 - dx * F2) * tf;
-      acc[5] += (dx * F1 - dy * F0) * tf;
-      acc[6] -= 0.5 * (dx * F0 + dx * F0) * tf;
-      acc[7] -= 0.5 * (dy * F1 + dy * F1) * tf;
-      acc[8] -= 0.5 * (dz * F2 + dz * F2) * tf;
-      acc[9] -= 0.5 * (dy * F0 + dx * F1) * tf;
-      acc[10] -= 0.5 * (dz * F0 + dx * F2) * tf;
-      acc[11] -= 0.5 * (dz * F1 + dy * F2) * tf;
+      for (int u = 0; u < U; ++u) {
+        if (sl[u] < 0) continue;
+        const int s = sl[u], n = nn[u];
+        const double F0 = a.Fh[(3 * s) * plane + n];
+        const double F1 = a.Fh[(3 * s + 1) * plane + n];
+        const double F2 = a.Fh[(3 * s + 2) * plane + n];
+        acc[0] += F0 * ff;
+        acc[1] += F1 * ff;
+        acc[2] += F2 * ff;
+        const double dx = (double)(z[0] + i0 + u + g.off[0]) - xL;
+        acc[3] += (dy * F2 - dz * F1) * tf;
+        acc[4] += (dz * F0 - dx * F2) * tf;
+        acc[5] += (dx * F1 - dy * F0) * tf;
+        acc[6] -= 0.5 * (dx * F0 + dx * F0) * tf;
+        acc[7] -= 0.5 * (dy * F1 + dy * F1) * tf;
+        acc[8] -= 0.5 * (dz * F2 + dz * F2) * tf;
+        acc[9] -= 0.5 * (dy * F0 + dx * F1) * tf;
+        acc[10] -= 0.5 * (dz * F0 + dx * F2) * tf;
+        acc[11] -= 0.5 * (dz * F1 + dy * F2) * tf;
+      }
     }
   }
 #pragma unroll
