@@ -221,6 +221,16 @@ LBM_HD int sf_count_columns(int dim, double dx, double dy, double dz, double rP_
 // cell of the column then receives the length of its overlap.
 // ---------------------------------------------------------------------------
 
+// Square root for the end-point prediction only (the exact tests decide):
+// the hardware approximation on the GPU.
+LBM_HD float sf_sqrt_approx(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
+
 // First index in [lo, hi] that passes, given that passing is monotone
 // non-decreasing on [lo, hi] and hi passes.
 LBM_HD int sf_first_pass(const double* Z, int lo, int hi, double s, double rSq) {
@@ -251,7 +261,7 @@ LBM_HD int sf_last_pass(const double* Z, int lo, int hi, double s, double rSq) {
 LBM_HD bool sf_line_interval(const double* Z, int P, int pstar, double t0, double s, double rSq,
                              int& lo, int& hi) {
   if (!(s + Z[pstar] < rSq)) return false;
-  const float h = sqrtf((float)(rSq - s));
+  const float h = sf_sqrt_approx((float)(rSq - s));
   const float x0 = (float)t0;
   int pe = (int)floorf((-h - x0) * 10.0f) + 1;
   pe = pe < 0 ? 0 : (pe > pstar ? pstar : pe);
