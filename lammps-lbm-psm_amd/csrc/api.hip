@@ -249,7 +249,6 @@ CollideArgs make_args(lbmpsm_ctx* c, int store_macros) {
   a.cb = c->d_cb;
   a.faces = c->d_faces;
   a.nfaces = c->nfaces;
-  a.host_faces = c->faces;
   return a;
 }
 
@@ -806,10 +805,6 @@ int lbmpsm_step(lbmpsm_ctx* c, int flags) {
   {
     ProfScope ps(c, "collide_stream");
     HIPCHK(c, launch_collide(c->cfg.storage, a, c->stream));
-  }
-  if (a.nfaces && !a.poststream) {
-    ProfScope ps(c, "collide_faces");
-    HIPCHK(c, launch_collide_faces(c->cfg.storage, a, c->stream));
   }
   c->cur = 1 - c->cur;
   c->poststream = 0;

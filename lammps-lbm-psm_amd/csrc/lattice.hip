@@ -248,9 +248,8 @@ __device__ __forceinline__ void gather_row(const CollideArgs& a, const Row& r, i
 }
 
 // True when node (i,j,k) lies on one of the faces (plane and ranges).
-__device__ __forceinline__ bool on_face(const Face* faces, int nfaces, int i, int j, int k,
-                                        int first_face_only_before) {
-  for (int fi = 0; fi < nfaces && fi < first_face_only_before; ++fi) {
+__device__ __forceinline__ bool on_face(const Face* faces, int nfaces, int i, int j, int k) {
+  for (int fi = 0; fi < nfaces; ++fi) {
     const Face& F = faces[fi];
     const int c = F.axis == 0 ? i : (F.axis == 1 ? j : k);
     if (c != F.plane) continue;
@@ -288,7 +287,10 @@ template <int DIM, int Q>
 __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, unsigned rb,
                                             unsigned uc);
 
-// Bulk: every node that is on no Zou-He face.
+// Bulk: every stored node, one wave per 64-node chunk of a row.  A node on
+// Zou-He faces has them applied, in table order, to its pulled populations;
+// its pulls are part of the row's coalesced loads (a z face node would
+// otherwise be a strided access of its own).
 template <int DIM, int Q>
 __global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_f64(const CollideArgs a) {
   const Geom& g = a.g;
@@ -296,11 +298,12 @@ __global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_f64(const CollideArg
   if (!make_row<DIM>(g, bulk_wave(), r)) return;
   const int c = r.c0 + (int)(threadIdx.x & 63);
   if (c >= r.ncols) return;
-  if (!a.poststream && a.nfaces &&
-      on_face(a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c, DIM == 3 ? c : 0, kMaxFaces))
-    return;
+  const bool face = !a.poststream && a.nfaces &&
+      on_face(a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c, DIM == 3 ? c : 0);
   double f[Q];
   gather_row<DIM, Q, ST_F64, false>(a, r, c, f);
+  if (__any(face)) apply_faces<Q, double>(f, a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c,
+                                          DIM == 3 ? c : 0, 0.0);
   collide_f64<DIM, Q>(a, f, row_base<DIM>(g, r, 1, 1), (unsigned)c);
 }
 
@@ -504,11 +507,12 @@ __global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_lowp(const CollideAr
   if (!make_row<DIM>(g, bulk_wave(), r)) return;
   const int c = r.c0 + (int)(threadIdx.x & 63);
   if (c >= r.ncols) return;
-  if (!a.poststream && a.nfaces &&
-      on_face(a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c, DIM == 3 ? c : 0, kMaxFaces))
-    return;
+  const bool face = !a.poststream && a.nfaces &&
+      on_face(a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c, DIM == 3 ? c : 0);
   float f[Q];
   gather_row<DIM, Q, ST, false>(a, r, c, f);
+  if (__any(face)) apply_faces<Q, float>(f, a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c,
+                                         DIM == 3 ? c : 0, (float)a.rho0);
   collide_lowp<DIM, Q, ST>(a, f, row_base<DIM>(g, r, 1, 1), (unsigned)c);
 }
 
@@ -656,43 +660,6 @@ __device__ __forceinline__ void gather(const CollideArgs& a, int i, int j, int k
   const int c = DIM == 3 ? k : j;
   make_row<DIM>(a.g, row * ((ncols + 63) / 64) + c / 64, r);
   gather_row<DIM, Q, ST, true>(a, r, c, f);
-}
-
-// Boundary: nodes on at least one face, each updated once with all its
-// faces applied in table order.  One launch covers every face: consecutive block ranges belong to faces in
-// application order.  A node shared by several faces is handled by the
-// first face that holds it; the faces only read the previous state, so
-// their order of execution does not matter.
-template <int DIM, int Q, int ST>
-__global__ __launch_bounds__(64) void k_collide_faces(const CollideArgs a) {
-  int fi = 0;
-  int64_t b0 = 0;
-  for (; fi < a.nfaces; ++fi) {
-    const Face& G = a.faces[fi];
-    const int64_t cnt = (int64_t)max(G.a_hi - G.a_lo + 1, 0) * max(G.b_hi - G.b_lo + 1, 0);
-    const int64_t nblk = (cnt + 63) / 64;
-    if ((int64_t)blockIdx.x < b0 + nblk) break;
-    b0 += nblk;
-  }
-  if (fi >= a.nfaces) return;
-  const Face F = a.faces[fi];
-  const int na = F.a_hi - F.a_lo + 1, nb = F.b_hi - F.b_lo + 1;
-  const int64_t t = ((int64_t)blockIdx.x - b0) * 64 + threadIdx.x;
-  if (na <= 0 || nb <= 0 || t >= (int64_t)na * nb) return;
-  const int ta = F.a_lo + (int)(t / nb), tb = F.b_lo + (int)(t % nb);
-  int i, j, k;
-  if (F.axis == 0) { i = F.plane; j = ta; k = tb; }
-  else if (F.axis == 1) { i = ta; j = F.plane; k = tb; }
-  else { i = ta; j = tb; k = F.plane; }
-  const Geom& g = a.g;
-  if (i < 0 || i >= g.sx || j < 0 || j >= g.sy || k < 0 || k >= g.sz) return;
-  if (on_face(a.faces, a.nfaces, i, j, k, fi)) return;
-  typename StoreT<ST>::C f[Q];
-  gather<DIM, Q, ST>(a, i, j, k, f);
-  const unsigned rb = DIM == 3 ? (unsigned)((i * g.sy + j) * g.sz) : (unsigned)(i * g.sy);
-  const unsigned uc = DIM == 3 ? (unsigned)k : (unsigned)j;
-  if constexpr (ST == ST_F64) collide_f64<DIM, Q>(a, f, rb, uc);
-  else collide_lowp<DIM, Q, ST>(a, f, rb, uc);
 }
 
 // ---------------------------------------------------------------------------
@@ -864,34 +831,8 @@ static hipError_t collide_dim(int storage, const CollideArgs& a, hipStream_t s) 
   return hipGetLastError();
 }
 
-template <int DIM, int Q>
-static hipError_t collide_faces_dim(int storage, const CollideArgs& a, hipStream_t s) {
-  if (a.poststream || a.nfaces <= 0) return hipSuccess;
-  int64_t blocks = 0;
-  for (int fi = 0; fi < a.nfaces; ++fi) {
-    const Face& F = a.host_faces[fi];
-    const int64_t cnt =
-        (int64_t)std::max(F.a_hi - F.a_lo + 1, 0) * std::max(F.b_hi - F.b_lo + 1, 0);
-    blocks += (cnt + 63) / 64;
-  }
-  if (blocks <= 0) return hipSuccess;
-  const dim3 fg((unsigned)blocks);
-  switch (storage) {
-    case ST_F64: hipLaunchKernelGGL((k_collide_faces<DIM, Q, ST_F64>), fg, dim3(64), 0, s, a); break;
-    case ST_F32: hipLaunchKernelGGL((k_collide_faces<DIM, Q, ST_F32>), fg, dim3(64), 0, s, a); break;
-    case ST_F16: hipLaunchKernelGGL((k_collide_faces<DIM, Q, ST_F16>), fg, dim3(64), 0, s, a); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
 hipError_t launch_collide(int storage, const CollideArgs& a, hipStream_t s) {
   return a.g.dim == 3 ? collide_dim<3, 19>(storage, a, s) : collide_dim<2, 9>(storage, a, s);
-}
-
-hipError_t launch_collide_faces(int storage, const CollideArgs& a, hipStream_t s) {
-  return a.g.dim == 3 ? collide_faces_dim<3, 19>(storage, a, s)
-                      : collide_faces_dim<2, 9>(storage, a, s);
 }
 
 template <int DIM, int Q>

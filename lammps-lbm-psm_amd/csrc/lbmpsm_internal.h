@@ -52,7 +52,6 @@ struct CollideArgs {
                         // halo, in storage form (device memory)
   const Face* faces;    // [nfaces] Zou-He faces in application order (device memory)
   int nfaces;
-  const Face* host_faces;  // the same table in host memory (launch sizes only)
 };
 
 // Workgroups per entry of the force/torque/stresslet reduction.
@@ -138,7 +137,6 @@ struct MapArgs {
 // Bulk nodes (every node on no Zou-He face) ...
 hipError_t launch_collide(int storage, const CollideArgs& a, hipStream_t s);
 // ... and the face nodes, with the faces applied after the pull.
-hipError_t launch_collide_faces(int storage, const CollideArgs& a, hipStream_t s);
 hipError_t launch_fill_planes(int storage, void* f, const Geom& g, const double* cb,
                               double rho0, float scale, hipStream_t s);
 // Post-stream + BC values of interior nodes into `out` (FP64, reference
