@@ -209,14 +209,17 @@ __device__ __forceinline__ T* lane_at(T* p, unsigned col) {
 // Pull the post-stream populations of the node in column c of row r, then
 // apply the Zou-He faces.  In post-stream mode the node's own values are
 // read and no face is applied (initial state / state set by the host).
-template <int DIM, int Q, int ST, bool FACES>
+// PS: 0 pull, 1 post-stream, 2 as a.poststream says (the bulk kernels are
+// compiled for one mode, which keeps the other path's loads out of their
+// wait-count bookkeeping).
+template <int DIM, int Q, int ST, bool FACES, int PS = 2>
 __device__ __forceinline__ void gather_row(const CollideArgs& a, const Row& r, int c,
                                            typename StoreT<ST>::C* f) {
   using T = typename StoreT<ST>::T;
   using C = typename StoreT<ST>::C;
   const Geom& g = a.g;
   const int64_t plane = g.plane;
-  if (a.poststream) {
+  if (PS == 1 || (PS == 2 && a.poststream)) {
     const unsigned rb = row_base<DIM>(g, r, 1, 1);
     Unroll<0, Q>::run([&]<int q>() {
       f[q] = ld<ST>(lane_at(pq_ptr<T>(a.fin, q, plane, rb), (unsigned)c), a.inv_scale);
@@ -282,40 +285,98 @@ __device__ __forceinline__ int bulk_wave() {
   return __builtin_amdgcn_readfirstlane(b * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
 }
 
+// Slot data of a node for the FP64 particle form: B = solid fraction, u_s =
+// rigid-body node velocity, Fa = node force carried over (zero for a slot
+// the map set this step).  psm: some lane of the wave holds a particle.
+struct SlotsF64 {
+  uint8_t m;
+  bool psm, o1, o2;
+  double B1, B2;
+  double us1[3], us2[3];
+  double Fa[6];
+};
+
+__device__ __forceinline__ void load_slots_f64(const CollideArgs& a, unsigned n, uint8_t m,
+                                               SlotsF64& S) {
+  const int64_t plane = a.g.plane;
+  S.m = m;
+  S.psm = __any(m != 0);
+  S.o1 = m & 1;
+  S.o2 = m & 2;
+  S.B1 = S.B2 = 0.0;
+  for (int d = 0; d < 3; ++d) S.us1[d] = S.us2[d] = 0.0;
+  for (int d = 0; d < 6; ++d) S.Fa[d] = 0.0;
+  if (!S.psm) return;
+  if (S.o1) {
+    S.B1 = a.sf[n];
+    S.us1[0] = a.us[n];
+    S.us1[1] = a.us[plane + n];
+    S.us1[2] = a.us[2 * plane + n];
+    if (!(m & kMaskFresh0)) {
+      S.Fa[0] = a.Fh[n];
+      S.Fa[1] = a.Fh[plane + n];
+      S.Fa[2] = a.Fh[2 * plane + n];
+    }
+  }
+  if (S.o2) {
+    S.B2 = a.sf[plane + n];
+    S.us2[0] = a.us[3 * plane + n];
+    S.us2[1] = a.us[4 * plane + n];
+    S.us2[2] = a.us[5 * plane + n];
+    if (!(m & kMaskFresh1)) {
+      S.Fa[3] = a.Fh[3 * plane + n];
+      S.Fa[4] = a.Fh[4 * plane + n];
+      S.Fa[5] = a.Fh[5 * plane + n];
+    }
+  }
+}
+
 // Collision of one node whose post-stream populations are in f[] (FP64).
 template <int DIM, int Q>
 __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, unsigned rb,
-                                            unsigned uc);
+                                            unsigned uc, SlotsF64& S);
 
 // Bulk: every stored node, one wave per 64-node chunk of a row.  A node on
 // Zou-He faces has them applied, in table order, to its pulled populations;
 // its pulls are part of the row's coalesced loads (a z face node would
 // otherwise be a strided access of its own).
-template <int DIM, int Q>
+template <int DIM, int Q, bool POST>
 __global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_f64(const CollideArgs a) {
   const Geom& g = a.g;
   Row r;
   if (!make_row<DIM>(g, bulk_wave(), r)) return;
   const int c = r.c0 + (int)(threadIdx.x & 63);
   if (c >= r.ncols) return;
-  const bool face = !a.poststream && a.nfaces &&
+  const bool face = !POST && a.nfaces &&
       on_face(a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c, DIM == 3 ? c : 0);
+  const unsigned rb = row_base<DIM>(g, r, 1, 1);
+  // the slot mask is loaded before the pulls and the slot data right after
+  // them: a wave with particle nodes has both in flight with the populations
+  const uint8_t m = a.mask[rb + (unsigned)c];
   double f[Q];
-  gather_row<DIM, Q, ST_F64, false>(a, r, c, f);
+  gather_row<DIM, Q, ST_F64, false, POST ? 1 : 0>(a, r, c, f);
+  SlotsF64 S;
+  load_slots_f64(a, rb + (unsigned)c, m, S);
   if (__any(face)) apply_faces<Q, double>(f, a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c,
                                           DIM == 3 ? c : 0, 0.0);
-  collide_f64<DIM, Q>(a, f, row_base<DIM>(g, r, 1, 1), (unsigned)c);
+  collide_f64<DIM, Q>(a, f, rb, (unsigned)c, S);
 }
 
 template <int DIM, int Q>
 __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, unsigned rb,
-                                            unsigned uc) {
+                                            unsigned uc, SlotsF64& S) {
   const Geom& g = a.g;
   const int64_t plane = g.plane;
   const double omega = a.omega, inv2 = a.invCs2, inv4 = a.invCs4;
   const double F0 = a.F[0], F1 = a.F[1], F2 = a.F[2];
   {
     const unsigned n = rb + uc;
+    const uint8_t m = S.m;
+    const bool psm = S.psm, o1 = S.o1, o2 = S.o2;
+    double B1 = S.B1, B2 = S.B2;
+    const double* us1 = S.us1;
+    const double* us2 = S.us2;
+    double* Fa = S.Fa;
     double rho, jx, jy, jz;
     moments<Q, double>(f, rho, jx, jy, jz);
     jx = jx + F0 * 0.5;
@@ -329,13 +390,12 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
       a.u_out[2 * plane + n] = uz;
     }
     const double usqt = 0.5 * (ux * ux + uy * uy + uz * uz) * inv2;
-    const uint8_t m = a.mask[n];
 
     // A wave with any particle node runs the particle form for all its
     // lanes: with both slots empty it yields the fluid value (the extra
     // terms add +0.0).  The branch is wave-uniform, so every population
     // store below is issued once by all 64 lanes together.
-    if (!__any(m != 0)) {
+    if (!psm) {
       Unroll<0, Q>::run([&]<int q>() {
         constexpr double w = VelSet<Q>::w[q];
         const double eu = edot<Q, q>(ux, uy, uz);
@@ -351,34 +411,6 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
         st<ST_F64>(lane_at(pq_ptr_w<double>(a.fout, q, plane, rb), uc), out, 1.0f);
       });
     } else {
-      // Occupied slots: B = solid fraction, u_s = rigid-body node velocity.
-      const bool o1 = m & 1, o2 = m & 2;
-      double B1 = o1 ? a.sf[n] : 0.0;
-      double B2 = o2 ? a.sf[plane + n] : 0.0;
-      double us1[3] = {0.0, 0.0, 0.0}, us2[3] = {0.0, 0.0, 0.0};
-      double Fa[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      // node forces continue from the stored value, or from zero for a slot
-      // the map set this step
-      if (o1) {
-        us1[0] = a.us[n];
-        us1[1] = a.us[plane + n];
-        us1[2] = a.us[2 * plane + n];
-        if (!(m & kMaskFresh0)) {
-          Fa[0] = a.Fh[n];
-          Fa[1] = a.Fh[plane + n];
-          Fa[2] = a.Fh[2 * plane + n];
-        }
-      }
-      if (o2) {
-        us2[0] = a.us[3 * plane + n];
-        us2[1] = a.us[4 * plane + n];
-        us2[2] = a.us[5 * plane + n];
-        if (!(m & kMaskFresh1)) {
-          Fa[3] = a.Fh[3 * plane + n];
-          Fa[4] = a.Fh[4 * plane + n];
-          Fa[5] = a.Fh[5 * plane + n];
-        }
-      }
       const double u1sq = 0.5 * (us1[0] * us1[0] + us1[1] * us1[1] + us1[2] * us1[2]) * inv2;
       const double u2sq = 0.5 * (us2[0] * us2[0] + us2[1] * us2[1] + us2[2] * us2[2]) * inv2;
       const double Btot0 = B1 + B2;
@@ -500,17 +532,17 @@ template <int DIM, int Q, typename Emit>
 __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f, unsigned n,
                                                   Emit&& emit);
 
-template <int DIM, int Q, int ST>
+template <int DIM, int Q, int ST, bool POST>
 __global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_lowp(const CollideArgs a) {
   const Geom& g = a.g;
   Row r;
   if (!make_row<DIM>(g, bulk_wave(), r)) return;
   const int c = r.c0 + (int)(threadIdx.x & 63);
   if (c >= r.ncols) return;
-  const bool face = !a.poststream && a.nfaces &&
+  const bool face = !POST && a.nfaces &&
       on_face(a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c, DIM == 3 ? c : 0);
   float f[Q];
-  gather_row<DIM, Q, ST, false>(a, r, c, f);
+  gather_row<DIM, Q, ST, false, POST ? 1 : 0>(a, r, c, f);
   if (__any(face)) apply_faces<Q, float>(f, a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c,
                                          DIM == 3 ? c : 0, (float)a.rho0);
   collide_lowp<DIM, Q, ST>(a, f, row_base<DIM>(g, r, 1, 1), (unsigned)c);
@@ -823,9 +855,18 @@ static hipError_t collide_dim(int storage, const CollideArgs& a, hipStream_t s) 
   const int wpb = LBMPSM_ROW_WAVES > 0 ? LBMPSM_ROW_WAVES : std::min((ncols + 63) / 64, 4);
   dim3 grid((unsigned)((waves + wpb - 1) / wpb)), block((unsigned)(64 * wpb));
   switch (storage) {
-    case ST_F64: hipLaunchKernelGGL((k_collide_f64<DIM, Q>), grid, block, 0, s, a); break;
-    case ST_F32: hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32>), grid, block, 0, s, a); break;
-    case ST_F16: hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16>), grid, block, 0, s, a); break;
+    case ST_F64:
+      if (a.poststream) hipLaunchKernelGGL((k_collide_f64<DIM, Q, true>), grid, block, 0, s, a);
+      else hipLaunchKernelGGL((k_collide_f64<DIM, Q, false>), grid, block, 0, s, a);
+      break;
+    case ST_F32:
+      if (a.poststream) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32, true>), grid, block, 0, s, a);
+      else hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32, false>), grid, block, 0, s, a);
+      break;
+    case ST_F16:
+      if (a.poststream) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16, true>), grid, block, 0, s, a);
+      else hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16, false>), grid, block, 0, s, a);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
