@@ -226,6 +226,8 @@ LBM_HD int sf_count_columns(int dim, double dx, double dy, double dz, double rP_
 LBM_HD float sf_sqrt_approx(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
   return __builtin_amdgcn_sqrtf(x);
+#elif defined(LBMPSM_SF_HOST_SQRT)
+  return LBMPSM_SF_HOST_SQRT(x);  // a host test's square root with a set error
 #else
   return sqrtf(x);
 #endif
@@ -254,23 +256,40 @@ LBM_HD int sf_last_pass(const double* Z, int lo, int hi, double s, double rSq) {
 }
 
 // The interval of passing points of one line.  The points' z offsets are
-// t_p ~ t_0 + p/10, so the real-arithmetic chord ends |t| < sqrt(r^2 - s)
-// predict both ends; each prediction is confirmed by its two exact tests
-// (the end passes, its outer neighbour fails) and otherwise the binary
-// search decides.  Returns false for an empty line.
+// t_p = t_0 + p/10 (up to ~1e-14), and a point passes the exact test
+// s + t_p^2 < r^2 when |t_p| < h = sqrt(r^2 - s) (the test's rounding moves
+// that bound by ~1e-14 of a point).  Both ends are predicted in float:
+// v_lo = 10 (-h - t_0) and v_hi = 10 (h - t_0); the first passing point is
+// floor(v_lo) + 1, the last floor(v_hi).  With e = 2^-24, the float values
+// differ from the real ones by at most 10 (h + |t_0|) e (2k + 3.5) for a
+// square root within k ulp (k = 2: 75 e (h + |t_0|)).  When v lies further
+// than tol = (h + |t_0|) 2^-16 (> 3x that bound) from an integer, its floor
+// is the exact one and the end needs no FP64 test.  Otherwise the end is
+// confirmed by its two exact tests (the end passes, its outer neighbour
+// fails), and failing those the binary search decides.  Returns false for
+// an empty line.
 LBM_HD bool sf_line_interval(const double* Z, int P, int pstar, double t0, double s, double rSq,
                              int& lo, int& hi) {
   if (!(s + Z[pstar] < rSq)) return false;
   const float h = sf_sqrt_approx((float)(rSq - s));
   const float x0 = (float)t0;
-  int pe = (int)floorf((-h - x0) * 10.0f) + 1;
+  const float tol = (h + fabsf(x0)) * (1.0f / 65536.0f);
+  const float vl = (-h - x0) * 10.0f, fl = floorf(vl);
+  int pe = (int)fl + 1;
   pe = pe < 0 ? 0 : (pe > pstar ? pstar : pe);
-  if (s + Z[pe] < rSq && (pe == 0 || !(s + Z[pe - 1] < rSq))) lo = pe;
-  else lo = sf_first_pass(Z, 0, pstar, s, rSq);
-  pe = (int)floorf((h - x0) * 10.0f);
+  const float rl = vl - fl;
+  if ((rl > tol && rl < 1.0f - tol) || (s + Z[pe] < rSq && (pe == 0 || !(s + Z[pe - 1] < rSq))))
+    lo = pe;
+  else
+    lo = sf_first_pass(Z, 0, pstar, s, rSq);
+  const float vh = (h - x0) * 10.0f, fh = floorf(vh);
+  pe = (int)fh;
   pe = pe < pstar ? pstar : (pe > P - 1 ? P - 1 : pe);
-  if (s + Z[pe] < rSq && (pe == P - 1 || !(s + Z[pe + 1] < rSq))) hi = pe;
-  else hi = sf_last_pass(Z, pstar, P - 1, s, rSq);
+  const float rh = vh - fh;
+  if ((rh > tol && rh < 1.0f - tol) || (s + Z[pe] < rSq && (pe == P - 1 || !(s + Z[pe + 1] < rSq))))
+    hi = pe;
+  else
+    hi = sf_last_pass(Z, pstar, P - 1, s, rSq);
   return true;
 }
 

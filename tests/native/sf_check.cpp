@@ -1,5 +1,19 @@
 // Host build of the library's solid-fraction routines, so the CPU tests can
-// compare them with the oracle's point-by-point count.
+// compare them with the oracle's point-by-point count.  Built a second time
+// with LBMPSM_SF_HOST_SQRT: the end-point prediction's float square root is
+// then off by a set number of ulps, larger than the GPU's, and the line form
+// must still give the exact counts.
+#ifdef LBMPSM_SF_HOST_SQRT
+#include <math.h>
+#include <stdlib.h>
+static int g_sqrt_ulps = 0;
+static float sf_sqrt_off(float x) {
+  float r = sqrtf(x);
+  for (int u = 0; u < abs(g_sqrt_ulps); ++u) r = nextafterf(r, g_sqrt_ulps > 0 ? INFINITY : 0.0f);
+  return r;
+}
+extern "C" void sf_set_sqrt_ulps(int k) { g_sqrt_ulps = k; }
+#endif
 #include "solid_fraction.h"
 
 extern "C" double sf_fast(int dim, double hsq_half, int i, int j, int k, double xP, double yP,

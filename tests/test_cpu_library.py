@@ -153,6 +153,51 @@ def test_line_form_counts_whole_zones(sfcheck):
     assert planes > 600
 
 
+def test_line_ends_hold_with_an_inexact_square_root():
+    """The line form predicts both ends of a line from a float square root
+    and skips the exact tests of an end that lies clear of a sub-point by
+    more than its error bound.  With that square root off by up to 4 ulp
+    (the GPU's is within 1-2), every counted cell still gets the per-cell
+    count: zones at random, with centres on a quarter grid, and with centres
+    and radii on the sub-point grid (crossings exactly on sub-points)."""
+    path = os.path.join(ROOT, "tests", "native", "libsf_check_ulp.so")
+    if not os.path.exists(path):
+        pytest.skip("tests/native not built")
+    lib = C.CDLL(path)
+    lib.sf_lines_plane.restype = C.c_int
+    lib.sf_lines_plane.argtypes = [C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.c_double, C.c_double, C.c_double, C.c_double]
+    lib.sf_set_sqrt_ulps.argtypes = [C.c_int]
+    hs = math.sqrt(3.1) / 2.0
+    rng = np.random.default_rng(57)
+    cases = []
+    for trial in range(24):
+        r = rng.uniform(0.6, 11.0)
+        c = rng.uniform(20.0, 21.0, 3)
+        if trial % 3 == 1:
+            c = np.round(c * 4) / 4
+        elif trial % 3 == 2:
+            c = np.round(c * 20) / 20
+            r = np.round(r * 20) / 20
+        cases.append((c, r))
+    planes = 0
+    try:
+        for ulps in (-4, -2, -1, 1, 2, 4):
+            lib.sf_set_sqrt_ulps(ulps)
+            for c, r in cases:
+                lo = (c - r).astype(int) - 2
+                hi = (c + r).astype(int) + 2
+                for i in range(lo[0], hi[0]):
+                    bad = lib.sf_lines_plane(hs, int(i), int(lo[1]), int(lo[2]),
+                                             int(hi[1] - lo[1]), int(hi[2] - lo[2]),
+                                             c[0], c[1], c[2], r)
+                    assert bad == 0, (ulps, c, r, i)
+                    planes += 1
+    finally:
+        lib.sf_set_sqrt_ulps(0)
+    assert planes > 1000
+
+
 def test_ghost_images_corner():
     box = H.Box(lo=np.zeros(3), prd=np.array([20.0, 20.0, 20.0]), periodic=(1, 1, 1))
     atoms = H.Atoms(tag=np.array([5], np.int32), x=np.array([[1.0, 19.0, 10.0]]),
