@@ -149,40 +149,26 @@ struct MapEntry {
 // a slot (the only case that uses them).
 struct NodeSlots {
   int t0, t1;
-  double sf0, sf1;
+  double sf0, sf1;  // values of the slots written this step
   bool d0, d1;      // slot written this step
-  bool k0, k1;      // sf0 / sf1 known (loaded or written)
 };
 
+// The reference's decision (src/LBM_PSM_exchangeParticleData.cpp:69-106)
+// is decFlag = (sf > 1e-5) + 2 (sf_old > 1e-5), sf_old being the fraction in
+// this tag's slot (0 when the tag holds none): 0 clear, 1 set, 2 clear if
+// the tag holds the slot, 3 set if sf > sf_old or the tag holds it.  A slot
+// is only ever set with sf > 1e-5 and cleared to tag 0, sf 0, so sf_old >
+// 1e-5 exactly when the tag holds a slot, and then cases 2 and 3 always act.
+// The table is therefore: sf > 1e-5 sets the tag's slot, anything else
+// clears it (a no-op when the tag holds none).  A tag 0 matching an empty
+// slot reads sf_old = 0 and takes cases 0 / 1, which act the same.  The old
+// fractions are never needed.
 __device__ __forceinline__ void map_transition(const MapArgs& a, const MapEntry& E, double sf,
                                                int ri, int rj, int rk, int n, NodeSlots& S) {
   const int64_t plane = a.g.plane;
   const int tag = E.tag;
-  int id_old = 0;
-  double sf_old = 0.0;
-  if (tag == S.t0) {
-    if (!S.k0) {
-      S.sf0 = a.s.sf[n];
-      S.k0 = true;
-    }
-    id_old = S.t0;
-    sf_old = S.sf0;
-  } else if (tag == S.t1) {
-    if (!S.k1) {
-      S.sf1 = a.s.sf[plane + n];
-      S.k1 = true;
-    }
-    id_old = S.t1;
-    sf_old = S.sf1;
-  }
-  const int dec = (sf > 0.00001) + 2 * (sf_old > 0.00001);
-  bool do_set = false, do_clear = false;
-  switch (dec) {
-    case 0: do_clear = true; break;
-    case 1: do_set = true; break;
-    case 2: do_clear = (id_old == tag); break;
-    case 3: do_set = (sf > sf_old || id_old == tag); break;
-  }
+  const bool do_set = sf > 0.00001;
+  const bool do_clear = !do_set;
   if (do_set) {  // setParticleOnLattice, src/LBM_PSM_lattice.cpp:257-299
     const double dx = (double)ri - E.xl[0];
     const double dy = (double)rj - E.xl[1];
@@ -201,9 +187,9 @@ __device__ __forceinline__ void map_transition(const MapArgs& a, const MapEntry&
     // after it); tag, solid fraction and force are stored at the end
     double* us = a.s.us + (sl == 1 ? 3 * plane : 0) + n;
     if (sl == 0) {
-      S.t0 = tag; S.sf0 = sf; S.k0 = true; S.d0 = true;
+      S.t0 = tag; S.sf0 = sf; S.d0 = true;
     } else if (sl == 1) {
-      S.t1 = tag; S.sf1 = sf; S.k1 = true; S.d1 = true;
+      S.t1 = tag; S.sf1 = sf; S.d1 = true;
     }
     if (sl >= 0) {
       us[0] = un[0];
@@ -213,10 +199,10 @@ __device__ __forceinline__ void map_transition(const MapArgs& a, const MapEntry&
   } else if (do_clear) {  // setToZero, src/LBM_PSM_lattice.cpp:302-324
     int sl = -1;
     if (S.t0 == tag) {
-      S.t0 = 0; S.sf0 = 0.0; S.k0 = true; S.d0 = true;
+      S.t0 = 0; S.sf0 = 0.0; S.d0 = true;
       sl = 0;
     } else if (S.t1 == tag) {
-      S.t1 = 0; S.sf1 = 0.0; S.k1 = true; S.d1 = true;
+      S.t1 = 0; S.sf1 = 0.0; S.d1 = true;
       sl = 1;
     }
     if (sl >= 0) {
@@ -521,14 +507,9 @@ __device__ __forceinline__ double map_code_sf(int dim, unsigned short cd) {
 }
 
 // Entries whose solid-fraction codes a node loads together before applying
-// them in order (their loads are independent, so they overlap), and whether
-// the solid fractions of occupied slots are loaded up front (1) or when an
-// entry of the same tag first needs them (0).
+// them in order (their loads are independent, so they overlap).
 #ifndef LBMPSM_MAP_BATCH
 #define LBMPSM_MAP_BATCH 2
-#endif
-#ifndef LBMPSM_MAP_EAGER
-#define LBMPSM_MAP_EAGER 1
 #endif
 constexpr int kMapBatch = LBMPSM_MAP_BATCH;
 #ifndef LBMPSM_MAP_WAVES
@@ -570,16 +551,7 @@ __device__ __forceinline__ void map_brick(const MapArgs& a, int64_t b, MapEntry*
   S.t0 = valid ? a.s.tag[n] : 0;
   S.t1 = valid ? a.s.tag[plane + n] : 0;
   S.d0 = S.d1 = false;
-#if LBMPSM_MAP_EAGER
-  // solid fractions of occupied slots are needed whenever an entry of the
-  // same tag arrives; load them now, alongside the list
-  S.sf0 = S.t0 != 0 ? a.s.sf[n] : 0.0;
-  S.sf1 = S.t1 != 0 ? a.s.sf[plane + n] : 0.0;
-  S.k0 = S.k1 = true;
-#else
   S.sf0 = S.sf1 = 0.0;
-  S.k0 = S.k1 = false;
-#endif
   // reference (halo'd) node indices, as used in the geometry
   const int ri = i + g.off[0], rj = j + g.off[1], rk = k + g.off[2];
 

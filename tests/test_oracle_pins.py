@@ -281,6 +281,36 @@ def test_slot_state_machine_scenarios():
     assert list(tag[10, 10, 10]) == [5, 3]
 
 
+def test_held_slots_always_exceed_the_set_threshold():
+    """The slot table of src/LBM_PSM_exchangeParticleData.cpp:81-106 only
+    ever sets a slot with sf > 1e-5 and clears it to tag 0, sf 0.  After
+    every step of random overlapping spheres that move, jump, vanish from the
+    list and return, a held slot has sf > 1e-5 and an empty one sf = 0: the
+    old fraction then only says whether the tag holds a slot, which is what
+    the GPU map relies on to skip reading it."""
+    n = (16, 16, 16)
+    box = lattice_box(3, n, (1, 1, 1))
+    o = O.Oracle(3, n, (1, 1, 1), 0.8)
+    rng = np.random.default_rng(404)
+    m = 7
+    pos = rng.uniform(0.0, 16.0, (m, 3))
+    radius = rng.uniform(1.2, 3.5, m)
+    held_seen = 0
+    for step in range(40):
+        pos = (pos + rng.uniform(-1.5, 1.5, (m, 3))) % 16.0
+        keep = rng.uniform(size=m) > 0.2
+        atoms = H.Atoms(tag=np.arange(1, m + 1, dtype=np.int32)[keep], x=pos[keep].copy(),
+                        v=np.zeros((keep.sum(), 3)), omega=np.zeros((keep.sum(), 3)),
+                        radius=radius[keep].copy())
+        o.fix_step(H.build_list(atoms, box, 6.0))
+        tag, sf, _, _ = o.slots()
+        held = tag != 0
+        assert np.all(sf[held] > 1e-5), step
+        assert np.all(sf[~held] == 0.0), step
+        held_seen += int(held.sum())
+    assert held_seen > 1000
+
+
 def test_hydro_known_answer():
     """One node force F at lever r: T = r x F, S = -sym(r F), summed with
     the force and torque factors (src/LBM_PSM_exchangeParticleData.cpp:223-240)."""
