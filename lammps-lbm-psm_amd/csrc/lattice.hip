@@ -485,9 +485,42 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
 // ---------------------------------------------------------------------------
 // FP32 arithmetic on shifted populations g = f - w*rho0 (F32 / F16 storage).
 // ---------------------------------------------------------------------------
+// Slot data of a node for the FP32 particle form (solid fractions and node
+// velocities in float); psm: some lane of the wave holds a particle.
+struct SlotsF32 {
+  uint8_t m;
+  bool psm, o1, o2;
+  float B1, B2;
+  float us1[3], us2[3];
+};
+
+__device__ __forceinline__ void load_slots_f32(const CollideArgs& a, unsigned n, uint8_t m,
+                                               SlotsF32& S) {
+  const int64_t plane = a.g.plane;
+  S.m = m;
+  S.psm = __any(m != 0);
+  S.o1 = m & 1;
+  S.o2 = m & 2;
+  S.B1 = S.B2 = 0.0f;
+  for (int d = 0; d < 3; ++d) S.us1[d] = S.us2[d] = 0.0f;
+  if (!S.psm) return;
+  if (S.o1) {
+    S.B1 = (float)a.sf[n];
+    S.us1[0] = (float)a.us[n];
+    S.us1[1] = (float)a.us[plane + n];
+    S.us1[2] = (float)a.us[2 * plane + n];
+  }
+  if (S.o2) {
+    S.B2 = (float)a.sf[plane + n];
+    S.us2[0] = (float)a.us[3 * plane + n];
+    S.us2[1] = (float)a.us[4 * plane + n];
+    S.us2[2] = (float)a.us[5 * plane + n];
+  }
+}
+
 template <int DIM, int Q, int ST>
 __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, unsigned rb,
-                                             unsigned uc);
+                                             unsigned uc, const SlotsF32& S);
 
 // Odd part of the FP32 equilibrium, 3 w_q (e_q . p) with p = rho u, formed
 // as a signed sum of A = 3 w_axis p per axis (w_axis = 1/18 in D3Q19, 1/9 in
@@ -530,7 +563,7 @@ __device__ __forceinline__ float eq_odd(float Ax, float Ay, float Az) {
 
 template <int DIM, int Q, typename Emit>
 __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f, unsigned n,
-                                                  Emit&& emit);
+                                                  const SlotsF32& S, Emit&& emit);
 
 template <int DIM, int Q, int ST, bool POST>
 __global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_lowp(const CollideArgs a) {
@@ -541,19 +574,24 @@ __global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_lowp(const CollideAr
   if (c >= r.ncols) return;
   const bool face = !POST && a.nfaces &&
       on_face(a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c, DIM == 3 ? c : 0);
+  const unsigned rb = row_base<DIM>(g, r, 1, 1);
+  // slot mask before the pulls, slot data right after them (as in FP64)
+  const uint8_t m = a.mask[rb + (unsigned)c];
   float f[Q];
   gather_row<DIM, Q, ST, false, POST ? 1 : 0>(a, r, c, f);
+  SlotsF32 S;
+  load_slots_f32(a, rb + (unsigned)c, m, S);
   if (__any(face)) apply_faces<Q, float>(f, a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c,
                                          DIM == 3 ? c : 0, (float)a.rho0);
-  collide_lowp<DIM, Q, ST>(a, f, row_base<DIM>(g, r, 1, 1), (unsigned)c);
+  collide_lowp<DIM, Q, ST>(a, f, rb, (unsigned)c, S);
 }
 
 template <int DIM, int Q, int ST>
 __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, unsigned rb,
-                                             unsigned uc) {
+                                             unsigned uc, const SlotsF32& S) {
   using T = typename StoreT<ST>::T;
   const int64_t plane = a.g.plane;
-  collide_lowp_node<DIM, Q>(a, f, rb + uc, [&](auto qc, float out) {
+  collide_lowp_node<DIM, Q>(a, f, rb + uc, S, [&](auto qc, float out) {
     st<ST>(lane_at(pq_ptr_w<T>(a.fout, decltype(qc)::value, plane, rb), uc), out, a.scale);
   });
 }
@@ -563,7 +601,7 @@ __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, uns
 // and the node's force goes to Fh.
 template <int DIM, int Q, typename Emit>
 __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f, unsigned n,
-                                                  Emit&& emit) {
+                                                  const SlotsF32& S, Emit&& emit) {
 #pragma clang fp contract(fast)
   const Geom& g = a.g;
   const int64_t plane = g.plane;
@@ -587,14 +625,14 @@ __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f
       a.u_out[2 * plane + n] = uz;
     }
     const float usq = 1.5f * (ux * ux + uy * uy + uz * uz);
-    const uint8_t m = a.mask[n];
+    const uint8_t m = S.m;
     // g_eq = w (drho + rho (4.5 (e.u)^2 - 1.5 u^2)) + 3 w e.(rho u), with
     // rho u = j + F/2 (u itself only enters terms even in e); mixed waves
     // run the particle form for all lanes
     // (wave-uniform branch, whole-wave stores)
     const float Ax = odd_axis_j<Q>(jx, hF0), Ay = odd_axis_j<Q>(jy, hF1),
                 Az = odd_axis_j<Q>(jz, hF2);
-    if (!__any(m != 0)) {
+    if (!S.psm) {
       Unroll<0, Q>::run([&]<int q>() {
         constexpr float w = (float)VelSet<Q>::w[q];
         const float eu = edot<Q, q>(ux, uy, uz);
@@ -608,21 +646,11 @@ __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f
         emit(std::integral_constant<int, q>{}, out);
       });
     } else {
-      const bool o1 = m & 1, o2 = m & 2;
-      float B1 = o1 ? (float)a.sf[n] : 0.0f;
-      float B2 = o2 ? (float)a.sf[plane + n] : 0.0f;
-      float us1[3] = {0.f, 0.f, 0.f}, us2[3] = {0.f, 0.f, 0.f};
+      const bool o1 = S.o1, o2 = S.o2;
+      float B1 = S.B1, B2 = S.B2;
+      const float* us1 = S.us1;
+      const float* us2 = S.us2;
       float Fa[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      if (o1) {
-        us1[0] = (float)a.us[n];
-        us1[1] = (float)a.us[plane + n];
-        us1[2] = (float)a.us[2 * plane + n];
-      }
-      if (o2) {
-        us2[0] = (float)a.us[3 * plane + n];
-        us2[1] = (float)a.us[4 * plane + n];
-        us2[2] = (float)a.us[5 * plane + n];
-      }
       const float u1sq = 1.5f * (us1[0] * us1[0] + us1[1] * us1[1] + us1[2] * us1[2]);
       const float u2sq = 1.5f * (us2[0] * us2[0] + us2[1] * us2[1] + us2[2] * us2[2]);
       float Btot = B1 + B2;
