@@ -213,8 +213,8 @@ def main():
     achieved = bytes_per_update * bulk_nodes / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_collide_f64<3,19>" if args.storage == "f64" else
-                      f"k_collide_lowp<3,19,{ {'f32': 1, 'f16': 2}[args.storage]}>",
+            "kernel": "k_collide_f64<3,19,false>" if args.storage == "f64" else
+                      f"k_collide_lowp<3,19,{ {'f32': 1, 'f16': 2}[args.storage]},false>",
             "bytes_per_update": bytes_per_update, "nodes_per_launch": int(bulk_nodes),
             "avg_launch_ms": round(avg_ms, 5)}
     # HBM bytes per launch of the dominant kernel from the committed
