@@ -206,6 +206,40 @@ __device__ __forceinline__ T* lane_at(T* p, unsigned col) {
   return (T*)((char*)p + col * (unsigned)sizeof(T));
 }
 
+// Storage index of direction q at stored node n.  FP64 keeps one plane per
+// direction (the layout of the adjoint's states and checkpoints).  FP32 and
+// FP16 keep blocks of 64 nodes that hold their Q directions as 64-element
+// runs: the loads and stores of a wave then touch one contiguous region per
+// node row instead of one per direction.
+#ifndef LBMPSM_LOWP_BLOCKED
+#define LBMPSM_LOWP_BLOCKED 1
+#endif
+template <int ST, int Q>
+__device__ __forceinline__ size_t pop_index(int q, int64_t plane, unsigned n) {
+  if constexpr (ST == ST_F64 || !LBMPSM_LOWP_BLOCKED) return (size_t)q * (size_t)plane + n;
+  else return ((size_t)(n >> 6) * Q + (size_t)q) * 64 + (n & 63u);
+}
+// Element of direction q at column col of the row starting at node rowb.
+template <int ST, int Q>
+__device__ __forceinline__ const typename StoreT<ST>::T* pop_ptr(const void* base, int q,
+                                                                 int64_t plane, unsigned rowb,
+                                                                 unsigned col) {
+  using T = typename StoreT<ST>::T;
+  if constexpr (ST == ST_F64 || !LBMPSM_LOWP_BLOCKED)
+    return lane_at(pq_ptr<T>(base, q, plane, rowb), col);
+  else
+    return (const T*)base + pop_index<ST, Q>(q, plane, rowb + col);
+}
+template <int ST, int Q>
+__device__ __forceinline__ typename StoreT<ST>::T* pop_ptr_w(void* base, int q, int64_t plane,
+                                                             unsigned rowb, unsigned col) {
+  using T = typename StoreT<ST>::T;
+  if constexpr (ST == ST_F64 || !LBMPSM_LOWP_BLOCKED)
+    return lane_at(pq_ptr_w<T>(base, q, plane, rowb), col);
+  else
+    return (T*)base + pop_index<ST, Q>(q, plane, rowb + col);
+}
+
 // Pull the post-stream populations of the node in column c of row r, then
 // apply the Zou-He faces.  In post-stream mode the node's own values are
 // read and no face is applied (initial state / state set by the host).
@@ -222,7 +256,7 @@ __device__ __forceinline__ void gather_row(const CollideArgs& a, const Row& r, i
   if (PS == 1 || (PS == 2 && a.poststream)) {
     const unsigned rb = row_base<DIM>(g, r, 1, 1);
     Unroll<0, Q>::run([&]<int q>() {
-      f[q] = ld<ST>(lane_at(pq_ptr<T>(a.fin, q, plane, rb), (unsigned)c), a.inv_scale);
+      f[q] = ld<ST>(pop_ptr<ST, Q>(a.fin, q, plane, rb, (unsigned)c), a.inv_scale);
     });
     return;
   }
@@ -239,8 +273,8 @@ __device__ __forceinline__ void gather_row(const CollideArgs& a, const Row& r, i
     const unsigned col = (unsigned)(ec == 1 ? cm : (ec == -1 ? cp : c));
     const bool cok = ec == 1 ? vcm : (ec == -1 ? vcp : true);
     const bool rok = (r.okmask >> (3 * (1 - ea) + (1 - eb))) & 1u;
-    const T* rowp = pq_ptr<T>(a.fin, q, plane, row_base<DIM>(g, r, 1 - ea, 1 - eb));
-    f[q] = (rok && cok) ? ld<ST>(lane_at(rowp, col), a.inv_scale) : (C)a.cb[q];
+    const T* src = pop_ptr<ST, Q>(a.fin, q, plane, row_base<DIM>(g, r, 1 - ea, 1 - eb), col);
+    f[q] = (rok && cok) ? ld<ST>(src, a.inv_scale) : (C)a.cb[q];
   });
   if constexpr (FACES) {
     const C r0 = (ST == ST_F64) ? C(0) : (C)a.rho0;
@@ -589,10 +623,9 @@ __global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_lowp(const CollideAr
 template <int DIM, int Q, int ST>
 __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, unsigned rb,
                                              unsigned uc, const SlotsF32& S) {
-  using T = typename StoreT<ST>::T;
   const int64_t plane = a.g.plane;
   collide_lowp_node<DIM, Q>(a, f, rb + uc, S, [&](auto qc, float out) {
-    st<ST>(lane_at(pq_ptr_w<T>(a.fout, decltype(qc)::value, plane, rb), uc), out, a.scale);
+    st<ST>(pop_ptr_w<ST, Q>(a.fout, decltype(qc)::value, plane, rb, uc), out, a.scale);
   });
 }
 
@@ -804,9 +837,10 @@ __global__ __launch_bounds__(256) void k_set_pop_ref(typename StoreT<ST>::T* f, 
   const int64_t n = ((int64_t)i * g.sy + j) * g.sz + k;
   Unroll<0, Q>::run([&]<int q>() {
     const double v = in[R * Q + q];
-    if constexpr (ST == ST_F64) f[q * g.plane + n] = v;
-    else if constexpr (ST == ST_F32) f[q * g.plane + n] = (float)(v - VelSet<Q>::w[q] * rho0);
-    else f[q * g.plane + n] = __float2half_rn((float)(v - VelSet<Q>::w[q] * rho0) * scale);
+    const size_t e = pop_index<ST, Q>(q, g.plane, (unsigned)n);
+    if constexpr (ST == ST_F64) f[e] = v;
+    else if constexpr (ST == ST_F32) f[e] = (float)(v - VelSet<Q>::w[q] * rho0);
+    else f[e] = __float2half_rn((float)(v - VelSet<Q>::w[q] * rho0) * scale);
   });
 }
 
@@ -838,15 +872,16 @@ __global__ __launch_bounds__(256) void k_mass(const CollideArgs a, double* parti
   if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
-// Fill population plane q with one value given in storage form.
-template <int ST>
+// Fill direction q of every stored node with one value given in storage form.
+template <int Q, int ST>
 __global__ __launch_bounds__(256) void k_fill(typename StoreT<ST>::T* f, int64_t plane, int q,
                                               double v, float scale) {
   const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (n >= plane) return;
-  if constexpr (ST == ST_F64) f[q * plane + n] = v;
-  else if constexpr (ST == ST_F32) f[q * plane + n] = (float)v;
-  else f[q * plane + n] = __float2half_rn((float)v * scale);
+  const size_t e = pop_index<ST, Q>(q, plane, (unsigned)n);
+  if constexpr (ST == ST_F64) f[e] = v;
+  else if constexpr (ST == ST_F32) f[e] = (float)v;
+  else f[e] = __float2half_rn((float)v * scale);
 }
 
 template <int Q, int ST>
@@ -862,9 +897,10 @@ __global__ __launch_bounds__(256) void k_set_pop(typename StoreT<ST>::T* f, Geom
   const int64_t o = (((int64_t)gi * g.ny + gj) * g.nz + gk) * Q;
   Unroll<0, Q>::run([&]<int q>() {
     const double v = in[o + q];
-    if constexpr (ST == ST_F64) f[q * g.plane + n] = v;
-    else if constexpr (ST == ST_F32) f[q * g.plane + n] = (float)(v - VelSet<Q>::w[q] * rho0);
-    else f[q * g.plane + n] = __float2half_rn((float)(v - VelSet<Q>::w[q] * rho0) * scale);
+    const size_t e = pop_index<ST, Q>(q, g.plane, (unsigned)n);
+    if constexpr (ST == ST_F64) f[e] = v;
+    else if constexpr (ST == ST_F32) f[e] = (float)(v - VelSet<Q>::w[q] * rho0);
+    else f[e] = __float2half_rn((float)(v - VelSet<Q>::w[q] * rho0) * scale);
   });
 }
 
@@ -980,19 +1016,26 @@ hipError_t launch_mass(int storage, const CollideArgs& a, double* partials, int 
 
 // cb holds one value per direction in storage form (raw for F64, shifted
 // for F32/F16).
-hipError_t launch_fill_planes(int storage, void* f, const Geom& g, const double* cb,
-                              double rho0, float scale, hipStream_t s) {
-  (void)rho0;
+template <int Q>
+static hipError_t fill_planes_q(int storage, void* f, const Geom& g, const double* cb,
+                                float scale, hipStream_t s) {
   dim3 grid((unsigned)((g.plane + 255) / 256));
-  for (int q = 0; q < g.q; ++q) {
+  for (int q = 0; q < Q; ++q) {
     switch (storage) {
-      case ST_F64: hipLaunchKernelGGL((k_fill<ST_F64>), grid, dim3(256), 0, s, (double*)f, g.plane, q, cb[q], scale); break;
-      case ST_F32: hipLaunchKernelGGL((k_fill<ST_F32>), grid, dim3(256), 0, s, (float*)f, g.plane, q, cb[q], scale); break;
-      case ST_F16: hipLaunchKernelGGL((k_fill<ST_F16>), grid, dim3(256), 0, s, (__half*)f, g.plane, q, cb[q], scale); break;
+      case ST_F64: hipLaunchKernelGGL((k_fill<Q, ST_F64>), grid, dim3(256), 0, s, (double*)f, g.plane, q, cb[q], scale); break;
+      case ST_F32: hipLaunchKernelGGL((k_fill<Q, ST_F32>), grid, dim3(256), 0, s, (float*)f, g.plane, q, cb[q], scale); break;
+      case ST_F16: hipLaunchKernelGGL((k_fill<Q, ST_F16>), grid, dim3(256), 0, s, (__half*)f, g.plane, q, cb[q], scale); break;
       default: return hipErrorInvalidValue;
     }
   }
   return hipGetLastError();
+}
+
+hipError_t launch_fill_planes(int storage, void* f, const Geom& g, const double* cb,
+                              double rho0, float scale, hipStream_t s) {
+  (void)rho0;
+  return g.q == 19 ? fill_planes_q<19>(storage, f, g, cb, scale, s)
+                   : fill_planes_q<9>(storage, f, g, cb, scale, s);
 }
 
 template <int Q>
