@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256) void k_adj_step(const AdjArgs a) {
     constexpr int ex = VelSet<Q>::ex[q], ey = VelSet<Q>::ey[q], ez = VelSet<Q>::ez[q];
     const int src = (wrapc(i - ex, g.sx) * g.sy + wrapc(j - ey, g.sy)) * g.sz + wrapc(k - ez, g.sz);
     const int dst = (wrapc(i + ex, g.sx) * g.sy + wrapc(j + ey, g.sy)) * g.sz + wrapc(k + ez, g.sz);
-    f[q] = a.hprev[q * P + (a.inplace ? n : src)];
+    f[q] = a.hprev[pop_index_f64<Q>(q, P, (unsigned)(a.inplace ? n : src))];
     mu[q] = a.lam_in ? a.lam_in[q * P + dst] : 0.0;
   });
 

@@ -206,26 +206,28 @@ __device__ __forceinline__ T* lane_at(T* p, unsigned col) {
   return (T*)((char*)p + col * (unsigned)sizeof(T));
 }
 
-// Storage index of direction q at stored node n.  FP64 keeps one plane per
-// direction (the layout of the adjoint's states and checkpoints).  FP32 and
-// FP16 keep blocks of 64 nodes that hold their Q directions as 64-element
-// runs: the loads and stores of a wave then touch one contiguous region per
-// node row instead of one per direction.
+// Storage index of direction q at stored node n: blocks of 64 nodes that
+// hold their Q directions as 64-element runs, so that the loads and stores
+// of a wave touch one contiguous region per node row instead of one per
+// direction (FP64: pop_index_f64, which the adjoint shares).
 #ifndef LBMPSM_LOWP_BLOCKED
 #define LBMPSM_LOWP_BLOCKED 1
 #endif
 template <int ST, int Q>
 __device__ __forceinline__ size_t pop_index(int q, int64_t plane, unsigned n) {
-  if constexpr (ST == ST_F64 || !LBMPSM_LOWP_BLOCKED) return (size_t)q * (size_t)plane + n;
+  if constexpr (ST == ST_F64) return pop_index_f64<Q>(q, plane, n);
+  else if constexpr (!LBMPSM_LOWP_BLOCKED) return (size_t)q * (size_t)plane + n;
   else return ((size_t)(n >> 6) * Q + (size_t)q) * 64 + (n & 63u);
 }
+template <int ST>
+constexpr bool kPlanar = ST == ST_F64 ? !LBMPSM_F64_BLOCKED : !LBMPSM_LOWP_BLOCKED;
 // Element of direction q at column col of the row starting at node rowb.
 template <int ST, int Q>
 __device__ __forceinline__ const typename StoreT<ST>::T* pop_ptr(const void* base, int q,
                                                                  int64_t plane, unsigned rowb,
                                                                  unsigned col) {
   using T = typename StoreT<ST>::T;
-  if constexpr (ST == ST_F64 || !LBMPSM_LOWP_BLOCKED)
+  if constexpr (kPlanar<ST>)
     return lane_at(pq_ptr<T>(base, q, plane, rowb), col);
   else
     return (const T*)base + pop_index<ST, Q>(q, plane, rowb + col);
@@ -234,7 +236,7 @@ template <int ST, int Q>
 __device__ __forceinline__ typename StoreT<ST>::T* pop_ptr_w(void* base, int q, int64_t plane,
                                                              unsigned rowb, unsigned col) {
   using T = typename StoreT<ST>::T;
-  if constexpr (ST == ST_F64 || !LBMPSM_LOWP_BLOCKED)
+  if constexpr (kPlanar<ST>)
     return lane_at(pq_ptr_w<T>(base, q, plane, rowb), col);
   else
     return (T*)base + pop_index<ST, Q>(q, plane, rowb + col);
@@ -442,7 +444,7 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
               w * (((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) * inv2 + (eu * inv4 * eF));
           out = out + a.guo_pref * Fi;
         }
-        st<ST_F64>(lane_at(pq_ptr_w<double>(a.fout, q, plane, rb), uc), out, 1.0f);
+        st<ST_F64>(pop_ptr_w<ST_F64, Q>(a.fout, q, plane, rb, uc), out, 1.0f);
       });
     } else {
       const double u1sq = 0.5 * (us1[0] * us1[0] + us1[1] * us1[1] + us1[2] * us1[2]) * inv2;
@@ -480,7 +482,7 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
               w * (((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) * inv2 + (eu * inv4 * eF));
           out = out + a.guo_pref * Fi;
         }
-        st<ST_F64>(lane_at(pq_ptr_w<double>(a.fout, q, plane, rb), uc), out, 1.0f);
+        st<ST_F64>(pop_ptr_w<ST_F64, Q>(a.fout, q, plane, rb, uc), out, 1.0f);
         // Node force on each slot, src/LBM_PSM_BGK_dynamics.h:101-113.
         if (o1) {
           const double t = -B1 * sc1;

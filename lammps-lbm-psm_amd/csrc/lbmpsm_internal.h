@@ -27,6 +27,20 @@ struct Geom {
   int64_t plane;   // nodes rounded up to a multiple of 64
 };
 
+// Index of direction q at stored node n in an FP64 population buffer of Q
+// directions and `plane` elements per direction: blocks of 64 nodes that hold
+// their Q directions as 64-element runs (LBMPSM_F64_BLOCKED = 1), or one plane
+// per direction (0).  The collide, the read-back kernels and the adjoint's
+// reads of recomputed states use it.
+#ifndef LBMPSM_F64_BLOCKED
+#define LBMPSM_F64_BLOCKED 1
+#endif
+template <int Q>
+__host__ __device__ inline size_t pop_index_f64(int q, int64_t plane, unsigned n) {
+  if constexpr (LBMPSM_F64_BLOCKED) return ((size_t)(n >> 6) * Q + (size_t)q) * 64 + (n & 63u);
+  else return (size_t)q * (size_t)plane + n;
+}
+
 // Pull-stream + BC + collide for one node.  fin/fout point at plane 0 of a
 // population buffer (element type depends on the storage form).
 struct CollideArgs {
