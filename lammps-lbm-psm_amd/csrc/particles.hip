@@ -398,6 +398,7 @@ struct SfLineShared {
   int part[kLineMax][kLineMax + 1];
   int diff[kLineMax][kLineMax + 1];
   int pstar;
+  int jlo, jhi;  // rows whose lines can reach the sphere
   double t0;   // z offset of the line's first point
 };
 
@@ -442,6 +443,10 @@ __global__ __launch_bounds__(256) void k_entry_sf_lines(const MapArgs a, int pla
     (&sh.part[0][0])[t] = 0;
     (&sh.diff[0][0])[t] = 0;
   }
+  if (tid == 0) {
+    sh.jlo = nj;
+    sh.jhi = -1;
+  }
   __syncthreads();
   // the line's minimum: along the V, the first point of the lowest value is
   // the only one below its predecessor and not above its successor
@@ -452,15 +457,34 @@ __global__ __launch_bounds__(256) void k_entry_sf_lines(const MapArgs a, int pla
   __syncthreads();
   const int pstar = sh.pstar;
   const double t0 = sh.t0;
+  // Rows none of whose lines can reach the sphere are skipped: a line of row
+  // j has s = X[a] + Y[j, b] >= minX + minY_j, and rounding is monotone, so
+  // when minX + minY_j + Z[pstar] >= r^2 every line of the row fails the
+  // first test of sf_line_interval.  Lines are enumerated over the rows
+  // [jlo, jhi] between the first and the last row that passes.
+  if (tid < nj) {
+    double mx = sh.X[0], my = sh.Y[kSfSlices * tid];
+    for (int c = 1; c < kSfSlices; ++c) {
+      mx = fmin(mx, sh.X[c]);
+      my = fmin(my, sh.Y[kSfSlices * tid + c]);
+    }
+    if (mx + my + sh.Z[pstar] < rSq) {
+      atomicMin(&sh.jlo, tid);
+      atomicMax(&sh.jhi, tid);
+    }
+  }
+  __syncthreads();
+  const int jlo = sh.jlo, nr = sh.jhi - sh.jlo + 1;  // nr <= 0: no line reaches
   // column fastest across lanes: lanes of a wave add into different columns
-  int ab = tid / nj, j = tid - ab * nj;
-  const int dab = 256 / nj, dj = 256 - dab * nj;
-  for (; ab < kSfPairs; ab += dab, j += dj) {
-    if (j >= nj) {
-      j -= nj;
+  int ab = nr > 0 ? tid / nr : kSfPairs, jr = nr > 0 ? tid - ab * nr : 0;
+  const int dab = nr > 0 ? 256 / nr : 0, dj = 256 - dab * nr;
+  for (; ab < kSfPairs; ab += dab, jr += dj) {
+    if (jr >= nr) {
+      jr -= nr;
       ++ab;
       if (ab >= kSfPairs) break;
     }
+    const int j = jlo + jr;
     const int ia = ab / kSfSlices, ib = ab - ia * kSfSlices;
     const double sxy = sh.X[ia] + sh.Y[kSfSlices * j + ib];
     int lo, hi;
