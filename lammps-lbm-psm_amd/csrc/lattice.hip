@@ -2,7 +2,8 @@
 // helpers that read the lattice back in the reference's form.
 //
 // State between steps is the post-collision population of every stored node
-// (structure of arrays, one plane per direction).  A step pulls
+// (blocks of 64 nodes, each holding its directions as 64-element runs;
+// see pop_index).  A step pulls
 // f_q(x) = S_q(x - e_q), applies the Zou-He faces of the previous step to
 // the pulled node, computes rho/u, collides and writes S'(x) to the other
 // buffer.  This is the reference's order (collide, push-stream, BC at the
@@ -188,7 +189,8 @@ __device__ __forceinline__ unsigned row_base(const Geom& g, const Row& r, int a,
   }
 }
 
-// Element p of plane q of a population buffer: uniform base, 32-bit offset.
+// Planar layout (one plane of `plane` elements per direction), kept for the
+// LBMPSM_*_BLOCKED = 0 builds: uniform base, 32-bit lane offset.
 template <typename T>
 __device__ __forceinline__ const T* pq_ptr(const void* base, int q, int64_t plane, unsigned rowb) {
   return (const T*)((const char*)base + ((size_t)q * (size_t)plane + rowb) * sizeof(T));
@@ -196,10 +198,6 @@ __device__ __forceinline__ const T* pq_ptr(const void* base, int q, int64_t plan
 template <typename T>
 __device__ __forceinline__ T* pq_ptr_w(void* base, int q, int64_t plane, unsigned rowb) {
   return (T*)((char*)base + ((size_t)q * (size_t)plane + rowb) * sizeof(T));
-}
-template <typename T>
-__device__ __forceinline__ T lane_ld(const T* p, unsigned col) {
-  return *(const T*)((const char*)p + col * (unsigned)sizeof(T));
 }
 template <typename T>
 __device__ __forceinline__ T* lane_at(T* p, unsigned col) {
