@@ -274,7 +274,14 @@ __device__ __forceinline__ void gather_row(const CollideArgs& a, const Row& r, i
     const bool cok = ec == 1 ? vcm : (ec == -1 ? vcp : true);
     const bool rok = (r.okmask >> (3 * (1 - ea) + (1 - eb))) & 1u;
     const T* src = pop_ptr<ST, Q>(a.fin, q, plane, row_base<DIM>(g, r, 1 - ea, 1 - eb), col);
-    f[q] = (rok && cok) ? ld<ST>(src, a.inv_scale) : (C)a.cb[q];
+    // both values unconditionally (the population from the buffer's first
+    // element for a pull from beyond a non-periodic halo), then a select:
+    // with no branch around the load, the loads of all directions are in
+    // flight together
+    const C cbv = (C)a.cb[q];
+    const bool ok = rok && cok;
+    const C v = ld<ST>(ok ? src : (const T*)a.fin, a.inv_scale);
+    f[q] = ok ? v : cbv;
   });
   if constexpr (FACES) {
     const C r0 = (ST == ST_F64) ? C(0) : (C)a.rho0;
