@@ -26,7 +26,10 @@
 #define LBMPSM_NT_STORE 1  // non-temporal population stores
 #endif
 #ifndef LBMPSM_NT_LOAD
-#define LBMPSM_NT_LOAD 0   // non-temporal population loads
+#define LBMPSM_NT_LOAD 1   // non-temporal population loads
+#endif
+#ifndef LBMPSM_F16_DWORD_LOAD
+#define LBMPSM_F16_DWORD_LOAD 0  // FP16: load the dword holding the element
 #endif
 #ifndef LBMPSM_WAVES_PER_EU
 #define LBMPSM_WAVES_PER_EU 0  // > 0: ask for this many waves per SIMD
@@ -70,7 +73,14 @@ template <int ST>
 __device__ __forceinline__ typename StoreT<ST>::C ld(const typename StoreT<ST>::T* p,
                                                       float inv_scale) {
   if constexpr (ST == ST_F16) {
-#if LBMPSM_NT_LOAD
+#if LBMPSM_F16_DWORD_LOAD
+    // the aligned dword that holds the element (neighbouring lanes load the
+    // same dword), then its half
+    const size_t ad = (size_t)p;
+    const unsigned w = *(const unsigned*)(ad & ~(size_t)3);
+    const unsigned short b = (unsigned short)((ad & 2) ? (w >> 16) : (w & 0xffffu));
+    return __half2float(__ushort_as_half(b)) * inv_scale;
+#elif LBMPSM_NT_LOAD
     const unsigned short b = __builtin_nontemporal_load((const unsigned short*)p);
     return __half2float(__ushort_as_half(b)) * inv_scale;
 #else
@@ -274,14 +284,12 @@ __device__ __forceinline__ void gather_row(const CollideArgs& a, const Row& r, i
     const bool cok = ec == 1 ? vcm : (ec == -1 ? vcp : true);
     const bool rok = (r.okmask >> (3 * (1 - ea) + (1 - eb))) & 1u;
     const T* src = pop_ptr<ST, Q>(a.fin, q, plane, row_base<DIM>(g, r, 1 - ea, 1 - eb), col);
-    // both values unconditionally (the population from the buffer's first
-    // element for a pull from beyond a non-periodic halo), then a select:
-    // with no branch around the load, the loads of all directions are in
-    // flight together
+    // both values unconditionally (src stays inside the buffer: nbr keeps
+    // the coordinate off a non-periodic edge), then a select: with no branch
+    // around the load, the loads of all directions are in flight together
     const C cbv = (C)a.cb[q];
-    const bool ok = rok && cok;
-    const C v = ld<ST>(ok ? src : (const T*)a.fin, a.inv_scale);
-    f[q] = ok ? v : cbv;
+    const C v = ld<ST>(src, a.inv_scale);
+    f[q] = (rok && cok) ? v : cbv;
   });
   if constexpr (FACES) {
     const C r0 = (ST == ST_F64) ? C(0) : (C)a.rho0;
