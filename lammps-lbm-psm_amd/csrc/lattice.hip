@@ -69,7 +69,8 @@ struct StoreT<ST_F16> {
   using C = float;
 };
 
-template <int ST>
+// NT: a non-temporal load (the line is not kept for reuse).
+template <int ST, bool NT = (LBMPSM_NT_LOAD != 0)>
 __device__ __forceinline__ typename StoreT<ST>::C ld(const typename StoreT<ST>::T* p,
                                                       float inv_scale) {
   if constexpr (ST == ST_F16) {
@@ -80,19 +81,18 @@ __device__ __forceinline__ typename StoreT<ST>::C ld(const typename StoreT<ST>::
     const unsigned w = *(const unsigned*)(ad & ~(size_t)3);
     const unsigned short b = (unsigned short)((ad & 2) ? (w >> 16) : (w & 0xffffu));
     return __half2float(__ushort_as_half(b)) * inv_scale;
-#elif LBMPSM_NT_LOAD
-    const unsigned short b = __builtin_nontemporal_load((const unsigned short*)p);
-    return __half2float(__ushort_as_half(b)) * inv_scale;
 #else
-    return __half2float(*p) * inv_scale;
+    if constexpr (NT) {
+      const unsigned short b = __builtin_nontemporal_load((const unsigned short*)p);
+      return __half2float(__ushort_as_half(b)) * inv_scale;
+    } else {
+      return __half2float(*p) * inv_scale;
+    }
 #endif
   } else {
     (void)inv_scale;
-#if LBMPSM_NT_LOAD
-    return __builtin_nontemporal_load(p);
-#else
-    return *p;
-#endif
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
   }
 }
 
