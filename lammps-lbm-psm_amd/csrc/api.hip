@@ -391,17 +391,35 @@ void build_faces(lbmpsm_ctx* c, int typeBC, double U, double rho_outlet) {
   }
 }
 
-int run_map(lbmpsm_ctx* c) {
+// The list is read from the context's packed block, or (dev non-null) from
+// the caller's device arrays, whose tags are then copied into the block.
+struct DevList {
+  const int* tag;
+  const double *x, *v, *w, *r;
+};
+
+int run_map(lbmpsm_ctx* c, const DevList* dev = nullptr) {
   ParticleDev& P = c->part;
   BrickDev& B = c->br;
   const int n = P.n;
   PrepArgs pa{};
   pa.g = c->g;
   pa.n = n;
-  pa.x = P.x(n);
-  pa.v = P.v(n);
-  pa.w = P.w(n);
-  pa.r = P.r(n);
+  if (dev) {
+    pa.x = dev->x;
+    pa.v = dev->v;
+    pa.w = dev->w;
+    pa.r = dev->r;
+    pa.tag_in = dev->tag;
+    pa.tag_out = (int*)P.tag(n);
+  } else {
+    pa.x = P.x(n);
+    pa.v = P.v(n);
+    pa.w = P.w(n);
+    pa.r = P.r(n);
+    pa.tag_in = P.tag(n);
+    pa.tag_out = nullptr;
+  }
   pa.xl = P.xl;
   pa.kin = P.kin;
   pa.zone = P.zone;
@@ -788,8 +806,8 @@ int lbmpsm_set_particles_device(lbmpsm_ctx* c, int n, const int* tag, const doub
   if (rc) return rc;
   rc = size_codes(c, n, max_radius);
   if (rc) return rc;
-  HIPCHK(c, launch_pack_particles(n, tag, x, v, omega, radius, c->part.raw, c->stream));
-  return run_map(c);
+  const DevList dev{tag, x, v, omega, radius};
+  return run_map(c, &dev);
 }
 
 int lbmpsm_step(lbmpsm_ctx* c, int flags) {

@@ -177,6 +177,8 @@ struct PrepArgs {
   const double* v;
   const double* w;
   const double* r;
+  const int* tag_in;  // the list's tags
+  int* tag_out;       // where the later kernels read them (null: tag_in is that copy)
   double* xl;
   double* kin;
   int* zone;
@@ -189,9 +191,6 @@ struct PrepArgs {
   int bi, bj, bk, nbi, nbj, nbk;
 };
 hipError_t launch_fill_int(int* p, int64_t n, int value, hipStream_t s);
-// Device-resident particle arrays into ParticleDev::raw.
-hipError_t launch_pack_particles(int n, const int* tag, const double* x, const double* v,
-                                 const double* w, const double* r, char* raw, hipStream_t s);
 // Per-entry lattice quantities and brick lists (one wavefront per entry).
 hipError_t launch_prep_bin(const PrepArgs& a, hipStream_t s);
 // Solid fraction of every entry at every node of its map zone.

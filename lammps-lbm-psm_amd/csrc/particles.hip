@@ -106,6 +106,7 @@ __global__ __launch_bounds__(64) void k_prep_bin(const PrepArgs a) {
   int z[6], rz[6];
   entry_geometry(a, p, loc, r_lb, z, rz);
   if (threadIdx.x == 0) {
+    if (a.tag_out) a.tag_out[p] = a.tag_in[p];
     a.xl[4 * p + 0] = loc[0];
     a.xl[4 * p + 1] = loc[1];
     a.xl[4 * p + 2] = loc[2];
@@ -144,9 +145,7 @@ struct MapEntry {
 
 // One state-machine transition of a node for one entry whose zone contains
 // it, given the entry's solid fraction at the node,
-// src/LBM_PSM_exchangeParticleData.cpp:69-106.  Solid fractions of the
-// node's slots are read from memory only when the entry's tag is found in
-// a slot (the only case that uses them).
+// src/LBM_PSM_exchangeParticleData.cpp:69-106.
 struct NodeSlots {
   int t0, t1;
   double sf0, sf1;  // values of the slots written this step
@@ -909,32 +908,6 @@ hipError_t launch_fill_int(int* p, int64_t n, int value, hipStream_t s) {
 hipError_t launch_prep_bin(const PrepArgs& a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_prep_bin, dim3(a.n), dim3(64), 0, s, a);
-  return hipGetLastError();
-}
-
-// Device-resident particle arrays into the packed per-call block
-// (x[n][3], v[n][3], omega[n][3], radius[n], tag[n]).
-__global__ __launch_bounds__(256) void k_pack_particles(int n, const int* tag, const double* x,
-                                                        const double* v, const double* w,
-                                                        const double* r, char* raw) {
-  const int t = (int)blockIdx.x * 256 + (int)threadIdx.x;
-  double* d = (double*)raw;
-  if (t < 3 * n) {
-    d[t] = x[t];
-    d[3 * n + t] = v[t];
-    d[6 * n + t] = w[t];
-  }
-  if (t < n) {
-    d[9 * n + t] = r[t];
-    ((int*)(raw + 80 * (size_t)n))[t] = tag[t];
-  }
-}
-
-hipError_t launch_pack_particles(int n, const int* tag, const double* x, const double* v,
-                                 const double* w, const double* r, char* raw, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pack_particles, dim3((unsigned)((3 * n + 255) / 256)), dim3(256), 0, s, n,
-                     tag, x, v, w, r, raw);
   return hipGetLastError();
 }
 
