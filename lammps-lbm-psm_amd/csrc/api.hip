@@ -34,6 +34,13 @@ struct lbmpsm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  // a particle list's upload, geometry and solid fractions run on stream2,
+  // beside the previous step's collide and reduction on `stream`; the map
+  // waits for them (ev_sf), and the next list's preparation waits for the map
+  // (ev_mapdone), whose brick lists and codes it reuses
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_sf = nullptr;
+  hipEvent_t ev_mapdone = nullptr;
 
   // populations: two buffers, cur holds the current state
   void* f[2] = {nullptr, nullptr};
@@ -132,24 +139,33 @@ hipEvent_t take_event(lbmpsm_ctx* c) {
 struct ProfScope {
   lbmpsm_ctx* c;
   const char* name;
+  hipStream_t st;
   hipEvent_t a = nullptr, b = nullptr;
   bool on = false;
-  ProfScope(lbmpsm_ctx* c_, const char* n) : c(c_), name(n) {
+  ProfScope(lbmpsm_ctx* c_, const char* n, hipStream_t s = nullptr)
+      : c(c_), name(n), st(s ? s : c_->stream) {
     // mode 1: every kernel; mode 2: the bulk collide only
     on = c->prof == 1 || (c->prof == 2 && std::strcmp(n, "collide_stream") == 0);
     if (on) {
       a = take_event(c);
       b = take_event(c);
-      if (a && b) (void)hipEventRecord(a, c->stream);
+      if (a && b) (void)hipEventRecord(a, st);
     }
   }
   ~ProfScope() {
     if (on && a && b) {
-      (void)hipEventRecord(b, c->stream);
+      (void)hipEventRecord(b, st);
       c->pending.push_back({name, a, b});
     }
   }
 };
+
+// Both streams idle (before buffers they may use are freed or reused).
+int sync_all(lbmpsm_ctx* c) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->stream2) HIPCHK(c, hipStreamSynchronize(c->stream2));
+  return LBMPSM_OK;
+}
 
 int prof_collect(lbmpsm_ctx* c) {
   if (c->pending.empty()) return LBMPSM_OK;
@@ -281,16 +297,22 @@ int ensure_particles(lbmpsm_ctx* c, int n) {
   ParticleDev& P = c->part;
   if (n <= P.cap) return LBMPSM_OK;
   // earlier steps may still read these buffers
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int rc = sync_all(c);
+  if (rc) return rc;
   int cap = std::max(n, std::max(64, P.cap * 2));
-  void* old[] = {P.raw, P.xl, P.kin, P.zone, P.rzone, P.out, P.hpart};
+  void* old[] = {P.raw, P.xl, P.kin, P.zone, P.rzone, P.out, P.hpart,
+                 P.raw_alt, P.xl_alt, P.kin_alt, P.zone_alt, P.rzone_alt};
   for (void* p : old) dev_free(p);
-  int rc = 0;
   rc |= dev_alloc(c, &P.raw, 84 * (size_t)cap);
   rc |= dev_alloc(c, &P.xl, 4 * (size_t)cap);
   rc |= dev_alloc(c, &P.kin, 6 * (size_t)cap);
   rc |= dev_alloc(c, &P.zone, 6 * (size_t)cap);
   rc |= dev_alloc(c, &P.rzone, 6 * (size_t)cap);
+  rc |= dev_alloc(c, &P.raw_alt, 84 * (size_t)cap);
+  rc |= dev_alloc(c, &P.xl_alt, 4 * (size_t)cap);
+  rc |= dev_alloc(c, &P.kin_alt, 6 * (size_t)cap);
+  rc |= dev_alloc(c, &P.zone_alt, 6 * (size_t)cap);
+  rc |= dev_alloc(c, &P.rzone_alt, 6 * (size_t)cap);
   rc |= dev_alloc(c, &P.out, 12 * (size_t)cap);
   rc |= dev_alloc(c, &P.hpart, (size_t)kHydroGroups * 12 * (size_t)cap);
   if (rc) return LBMPSM_ERR_NOMEM;
@@ -315,7 +337,8 @@ int size_codes(lbmpsm_ctx* c, int n, double rmax) {
   cap = std::min<int64_t>(cap, (int64_t)1 << 21);
   const size_t need = (size_t)cap * (size_t)n;
   if (need > c->sfcode_elems) {
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    rc = sync_all(c);
+    if (rc) return rc;
     const size_t alloc = std::max(need, c->sfcode_elems * 2);
     dev_free(c->sfcode);
     c->device_bytes -= (int64_t)(c->sfcode_elems * sizeof(unsigned short));
@@ -439,9 +462,12 @@ int run_map(lbmpsm_ctx* c, const DevList* dev = nullptr) {
   pa.nbi = B.nbi;
   pa.nbj = B.nbj;
   pa.nbk = B.nbk;
+  // this list's preparation reuses the brick lists, codes and work-list
+  // counter of the previous map: it waits for it (the upload, if any, is
+  // already queued behind the same wait)
   {
-    ProfScope ps(c, "map_bin");  // counts are zero: set at create, reset by k_map
-    HIPCHK(c, launch_prep_bin(pa, c->stream));
+    ProfScope ps(c, "map_bin", c->stream2);  // counts are zero: set at create, reset by k_map
+    HIPCHK(c, launch_prep_bin(pa, c->stream2));
   }
   MapArgs ma{};
   ma.g = c->g;
@@ -467,13 +493,30 @@ int run_map(lbmpsm_ctx* c, const DevList* dev = nullptr) {
   ma.slow_list = B.slow_list;
   ma.slow_count = B.slow_count;
   {
-    ProfScope ps(c, "map_fraction");
-    HIPCHK(c, launch_entry_sf(ma, c->stream));
+    ProfScope ps(c, "map_fraction", c->stream2);
+    HIPCHK(c, launch_entry_sf(ma, c->stream2));
   }
+  HIPCHK(c, hipEventRecord(c->ev_sf, c->stream2));
+  HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_sf, 0));
   {
     ProfScope ps(c, "map_particles");
     HIPCHK(c, launch_map(ma, B.nbricks, c->stream));
   }
+  HIPCHK(c, hipEventRecord(c->ev_mapdone, c->stream));
+  return LBMPSM_OK;
+}
+
+// A new list goes into the arrays of the list before the current one (whose
+// step is complete once the current list's map is): swap the two sets and
+// order stream2 after that map.
+int next_list(lbmpsm_ctx* c) {
+  ParticleDev& P = c->part;
+  std::swap(P.raw, P.raw_alt);
+  std::swap(P.xl, P.xl_alt);
+  std::swap(P.kin, P.kin_alt);
+  std::swap(P.zone, P.zone_alt);
+  std::swap(P.rzone, P.rzone_alt);
+  HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_mapdone, 0));
   return LBMPSM_OK;
 }
 
@@ -616,6 +659,12 @@ int lbmpsm_create(const lbmpsm_config* cfg, lbmpsm_ctx** out) {
     delete c;
     return fail(nullptr, LBMPSM_ERR_HIP, "hipStreamCreate failed");
   }
+  if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_sf, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_mapdone, hipEventDisableTiming) != hipSuccess) {
+    lbmpsm_destroy(c);
+    return fail(nullptr, LBMPSM_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
+  }
   int rc = 0;
   const size_t P = (size_t)g.plane;
   rc |= dev_alloc(c, (char**)&c->f[0], P * g.q * c->elem);
@@ -671,6 +720,7 @@ int lbmpsm_destroy(lbmpsm_ctx* c) {
   if (!c) return LBMPSM_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   dev_free(c->f[0]);
   dev_free(c->f[1]);
   dev_free(c->slots.tag);
@@ -681,7 +731,8 @@ int lbmpsm_destroy(lbmpsm_ctx* c) {
   dev_free(c->rho);
   dev_free(c->u);
   ParticleDev& P = c->part;
-  void* pp[] = {P.raw, P.xl, P.kin, P.zone, P.rzone, P.out, P.hpart};
+  void* pp[] = {P.raw, P.xl, P.kin, P.zone, P.rzone, P.out, P.hpart,
+                P.raw_alt, P.xl_alt, P.kin_alt, P.zone_alt, P.rzone_alt};
   for (void* p : pp) dev_free(p);
   BrickDev& B = c->br;
   void* bb[] = {B.count, B.list, B.slow_list, B.slow_count};
@@ -703,6 +754,9 @@ int lbmpsm_destroy(lbmpsm_ctx* c) {
   }
   for (auto e : c->free_events) (void)hipEventDestroy(e);
   if (c->stage_done) (void)hipEventDestroy(c->stage_done);
+  if (c->ev_sf) (void)hipEventDestroy(c->ev_sf);
+  if (c->ev_mapdone) (void)hipEventDestroy(c->ev_mapdone);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return LBMPSM_OK;
@@ -776,6 +830,8 @@ int lbmpsm_set_particles(lbmpsm_ctx* c, int n, const int* tag, const double* x, 
   }
   rc = size_codes(c, n, rmax);
   if (rc) return rc;
+  rc = next_list(c);
+  if (rc) return rc;
   // stage into pinned memory (after the previous upload has left it), one copy
   HIPCHK(c, hipEventSynchronize(c->stage_done));
   char* h = (char*)c->h_stage;
@@ -785,8 +841,8 @@ int lbmpsm_set_particles(lbmpsm_ctx* c, int n, const int* tag, const double* x, 
   std::memcpy(hd + 6 * (size_t)n, omega, 3 * (size_t)n * sizeof(double));
   std::memcpy(hd + 9 * (size_t)n, radius, (size_t)n * sizeof(double));
   std::memcpy(h + 80 * (size_t)n, tag, (size_t)n * sizeof(int));
-  HIPCHK(c, hipMemcpyAsync(c->part.raw, h, 84 * (size_t)n, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipEventRecord(c->stage_done, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->part.raw, h, 84 * (size_t)n, hipMemcpyHostToDevice, c->stream2));
+  HIPCHK(c, hipEventRecord(c->stage_done, c->stream2));
   return run_map(c);
 }
 
@@ -805,6 +861,8 @@ int lbmpsm_set_particles_device(lbmpsm_ctx* c, int n, const int* tag, const doub
   int rc = ensure_particles(c, n);
   if (rc) return rc;
   rc = size_codes(c, n, max_radius);
+  if (rc) return rc;
+  rc = next_list(c);
   if (rc) return rc;
   const DevList dev{tag, x, v, omega, radius};
   return run_map(c, &dev);

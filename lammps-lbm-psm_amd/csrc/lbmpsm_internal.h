@@ -91,6 +91,13 @@ struct ParticleDev {
   double* kin = nullptr;    // [n][6]  uTrans, omegaLB
   int* zone = nullptr;      // [n][6]  stored-coordinate map zone [lo,hi) x,y,z
   int* rzone = nullptr;     // [n][6]  stored-coordinate reduction zone
+  // the same arrays of the previous list: a new list is prepared in them
+  // while the step of the previous one may still read its own
+  char* raw_alt = nullptr;
+  double* xl_alt = nullptr;
+  double* kin_alt = nullptr;
+  int* zone_alt = nullptr;
+  int* rzone_alt = nullptr;
   double* out = nullptr;    // [n][12] F, T, S (physical units)
   double* hpart = nullptr;  // [n][kHydroGroups][12] group sums of the reduction
   const double* x(int nn) const { return (const double*)raw; }
