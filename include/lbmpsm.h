@@ -3,13 +3,14 @@
  * `fix lbm-psm` and `fix lbm-psm-bc`.
  *
  * One context holds one lattice resident in device memory (populations,
- * per-node particle slots, macroscopic fields) and one HIP stream.  All
- * arguments are plain pointers and sizes; host arrays belong to the caller
- * and are only read or written during the call.  Every entry point returns
- * LBMPSM_OK (0) or a negative error code and never throws or exits;
- * lbmpsm_last_error() gives the message.  One context per host thread;
- * work is ordered on the context's stream and the lbmpsm_get_* calls
- * synchronise it.
+ * per-node particle slots, macroscopic fields) and two HIP streams: the
+ * main one, and one on which a new particle list is prepared (upload,
+ * geometry, solid fractions) beside the previous step.  All arguments are
+ * plain pointers and sizes; host arrays belong to the caller and are only
+ * read or written during the call.  Every entry point returns LBMPSM_OK (0)
+ * or a negative error code and never throws or exits; lbmpsm_last_error()
+ * gives the message.  One context per host thread; the calls' work is
+ * ordered as the calls are made, and the lbmpsm_get_* calls synchronise it.
  *
  * Coordinates, velocities, angular velocities, radii, forces, torques and
  * stresslets cross the boundary in the fix's physical units, exactly as
@@ -127,7 +128,10 @@ int lbmpsm_set_particles(lbmpsm_ctx* ctx, int n, const int* tag, const double* x
  * pointers with the same layouts, and max_radius bounds the radii (it sizes
  * the per-entry solid-fraction tables; an entry above it is still mapped
  * exactly, by the slower in-kernel path).  No host transfer and no
- * synchronisation.  No counterpart in the reference: for callers whose
+ * synchronisation: the arrays are read by device work queued by this call,
+ * so they must hold the list when the call is made and stay unchanged until
+ * the next lbmpsm_step has been synchronised (any lbmpsm_get_* or
+ * lbmpsm_sync).  No counterpart in the reference: for callers whose
  * particle data live on the GPU. */
 int lbmpsm_set_particles_device(lbmpsm_ctx* ctx, int n, const int* tag, const double* x,
                                 const double* v, const double* omega, const double* radius,
