@@ -245,6 +245,9 @@ CollideArgs make_args(lbmpsm_ctx* c, int store_macros) {
   a.sf = c->slots.sf;
   a.us = c->slots.us;
   a.Fh = c->slots.Fh;
+  a.sff = c->slots.sff;
+  a.usf = c->slots.usf;
+  a.Fhf = c->slots.Fhf;
   a.rho_out = store_macros ? c->rho : nullptr;
   a.u_out = store_macros ? c->u : nullptr;
   a.poststream = c->poststream;
@@ -674,6 +677,11 @@ int lbmpsm_create(const lbmpsm_config* cfg, lbmpsm_ctx** out) {
   rc |= dev_alloc(c, &c->slots.us, 6 * P);
   rc |= dev_alloc(c, &c->slots.Fh, 6 * P);
   rc |= dev_alloc(c, &c->slots.mask, P);
+  if (c->cfg.storage != LBMPSM_STORE_F64) {
+    c->slots.sff = reinterpret_cast<float*>(c->slots.sf);
+    c->slots.usf = reinterpret_cast<float*>(c->slots.us);
+    c->slots.Fhf = reinterpret_cast<float*>(c->slots.Fh);
+  }
   rc |= dev_alloc(c, &c->d_cb, 19);
   rc |= dev_alloc(c, &c->d_faces, kMaxFaces);
   // node bricks for the particle map: 256 nodes each
@@ -893,6 +901,7 @@ int lbmpsm_step(lbmpsm_ctx* c, int flags) {
   h.rzone = c->part.rzone;
   h.slot_tag = c->slots.tag;
   h.Fh = c->slots.Fh;
+  h.Fhf = c->slots.Fhf;
   h.forceFactor = c->cfg.force_factor;
   h.torqueFactor = c->cfg.torque_factor;
   h.part = c->part.hpart;

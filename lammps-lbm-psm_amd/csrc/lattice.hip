@@ -554,16 +554,16 @@ __device__ __forceinline__ void load_slots_f32(const CollideArgs& a, unsigned n,
   for (int d = 0; d < 3; ++d) S.us1[d] = S.us2[d] = 0.0f;
   if (!S.psm) return;
   if (S.o1) {
-    S.B1 = (float)a.sf[n];
-    S.us1[0] = (float)a.us[n];
-    S.us1[1] = (float)a.us[plane + n];
-    S.us1[2] = (float)a.us[2 * plane + n];
+    S.B1 = a.sff[n];
+    S.us1[0] = a.usf[n];
+    S.us1[1] = a.usf[plane + n];
+    S.us1[2] = a.usf[2 * plane + n];
   }
   if (S.o2) {
-    S.B2 = (float)a.sf[plane + n];
-    S.us2[0] = (float)a.us[3 * plane + n];
-    S.us2[1] = (float)a.us[4 * plane + n];
-    S.us2[2] = (float)a.us[5 * plane + n];
+    S.B2 = a.sff[plane + n];
+    S.us2[0] = a.usf[3 * plane + n];
+    S.us2[1] = a.usf[4 * plane + n];
+    S.us2[2] = a.usf[5 * plane + n];
   }
 }
 
@@ -743,15 +743,15 @@ __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f
       // the map set this step
       if (o1) {
         const bool fr = m & kMaskFresh0;
-        a.Fh[n] = (fr ? 0.0 : a.Fh[n]) + (double)Fa[0];
-        a.Fh[plane + n] = (fr ? 0.0 : a.Fh[plane + n]) + (double)Fa[1];
-        a.Fh[2 * plane + n] = (fr ? 0.0 : a.Fh[2 * plane + n]) + (double)Fa[2];
+        a.Fhf[n] = (fr ? 0.0f : a.Fhf[n]) + Fa[0];
+        a.Fhf[plane + n] = (fr ? 0.0f : a.Fhf[plane + n]) + Fa[1];
+        a.Fhf[2 * plane + n] = (fr ? 0.0f : a.Fhf[2 * plane + n]) + Fa[2];
       }
       if (o2) {
         const bool fr = m & kMaskFresh1;
-        a.Fh[3 * plane + n] = (fr ? 0.0 : a.Fh[3 * plane + n]) + (double)Fa[3];
-        a.Fh[4 * plane + n] = (fr ? 0.0 : a.Fh[4 * plane + n]) + (double)Fa[4];
-        a.Fh[5 * plane + n] = (fr ? 0.0 : a.Fh[5 * plane + n]) + (double)Fa[5];
+        a.Fhf[3 * plane + n] = (fr ? 0.0f : a.Fhf[3 * plane + n]) + Fa[3];
+        a.Fhf[4 * plane + n] = (fr ? 0.0f : a.Fhf[4 * plane + n]) + Fa[4];
+        a.Fhf[5 * plane + n] = (fr ? 0.0f : a.Fhf[5 * plane + n]) + Fa[5];
       }
       if (m & (kMaskFresh0 | kMaskFresh1)) a.mask[n] = m & (kMaskOcc0 | kMaskOcc1);
     }

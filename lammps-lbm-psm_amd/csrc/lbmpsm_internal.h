@@ -52,6 +52,9 @@ struct CollideArgs {
   const double* sf;     // [2][plane]
   const double* us;     // [6][plane]   (3*s + d)
   double* Fh;           // [6][plane]   (3*s + d), lattice units
+  const float* sff;     // sf, us, Fh in float (FP32 / FP16 storage)
+  const float* usf;
+  float* Fhf;
   double* rho_out;      // [plane]      or null
   double* u_out;        // [3][plane]   or null
   int poststream;       // 1: fin already holds post-stream values (no pull, no BC)
@@ -127,6 +130,12 @@ struct SlotsDev {
   double* us = nullptr;     // [6][plane]
   double* Fh = nullptr;     // [6][plane]
   uint8_t* mask = nullptr;  // [plane]
+  // FP32 / FP16 storage: solid fractions, node velocities and node forces
+  // are kept in float, in the first half of sf / us / Fh (null for FP64
+  // storage)
+  float* sff = nullptr;
+  float* usf = nullptr;
+  float* Fhf = nullptr;
 };
 
 // Solid-fraction code of an entry at a node of its map zone: the
@@ -212,6 +221,7 @@ struct HydroArgs {
   const int* rzone;
   const int* slot_tag;
   const double* Fh;
+  const float* Fhf;  // node forces in float (FP32 / FP16 storage), else null
   double forceFactor, torqueFactor;
   double* part;  // [n][kHydroGroups][12] group sums
   double* out;

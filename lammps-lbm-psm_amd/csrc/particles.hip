@@ -162,6 +162,22 @@ struct NodeSlots {
 // clears it (a no-op when the tag holds none).  A tag 0 matching an empty
 // slot reads sf_old = 0 and takes cases 0 / 1, which act the same.  The old
 // fractions are never needed.
+// Stores a node vector (three components, plane apart) at element o of the
+// double array, or of its float view when the storage keeps node vectors in
+// float.
+__device__ __forceinline__ void store_node_vec(double* d, float* f, int64_t o, int64_t plane,
+                                               double x, double y, double z) {
+  if (f) {
+    f[o] = (float)x;
+    f[o + plane] = (float)y;
+    f[o + 2 * plane] = (float)z;
+  } else {
+    d[o] = x;
+    d[o + plane] = y;
+    d[o + 2 * plane] = z;
+  }
+}
+
 __device__ __forceinline__ void map_transition(const MapArgs& a, const MapEntry& E, double sf,
                                                int ri, int rj, int rk, int n, NodeSlots& S) {
   const int64_t plane = a.g.plane;
@@ -184,17 +200,12 @@ __device__ __forceinline__ void map_transition(const MapArgs& a, const MapEntry&
     else if (S.t1 == 0) sl = 1;
     // the node velocity is stored at once (a later entry of the node stores
     // after it); tag, solid fraction and force are stored at the end
-    double* us = a.s.us + (sl == 1 ? 3 * plane : 0) + n;
     if (sl == 0) {
       S.t0 = tag; S.sf0 = sf; S.d0 = true;
     } else if (sl == 1) {
       S.t1 = tag; S.sf1 = sf; S.d1 = true;
     }
-    if (sl >= 0) {
-      us[0] = un[0];
-      us[plane] = un[1];
-      us[2 * plane] = un[2];
-    }
+    if (sl >= 0) store_node_vec(a.s.us, a.s.usf, 3 * plane * sl + n, plane, un[0], un[1], un[2]);
   } else if (do_clear) {  // setToZero, src/LBM_PSM_lattice.cpp:302-324
     int sl = -1;
     if (S.t0 == tag) {
@@ -204,12 +215,7 @@ __device__ __forceinline__ void map_transition(const MapArgs& a, const MapEntry&
       S.t1 = 0; S.sf1 = 0.0; S.d1 = true;
       sl = 1;
     }
-    if (sl >= 0) {
-      double* us = a.s.us + 3 * plane * sl + n;
-      us[0] = 0.0;
-      us[plane] = 0.0;
-      us[2 * plane] = 0.0;
-    }
+    if (sl >= 0) store_node_vec(a.s.us, a.s.usf, 3 * plane * sl + n, plane, 0.0, 0.0, 0.0);
   }
 }
 
@@ -681,21 +687,15 @@ __device__ __forceinline__ void map_brick(const MapArgs& a, int64_t b, MapEntry*
   // a slot cleared this step (setToZero) keeps no force
   if (S.d0) {
     a.s.tag[n] = S.t0;
-    a.s.sf[n] = S.sf0;
-    if (S.t0 <= 0) {
-      a.s.Fh[n] = 0.0;
-      a.s.Fh[plane + n] = 0.0;
-      a.s.Fh[2 * plane + n] = 0.0;
-    }
+    if (a.s.sff) a.s.sff[n] = (float)S.sf0;
+    else a.s.sf[n] = S.sf0;
+    if (S.t0 <= 0) store_node_vec(a.s.Fh, a.s.Fhf, n, plane, 0.0, 0.0, 0.0);
   }
   if (S.d1) {
     a.s.tag[plane + n] = S.t1;
-    a.s.sf[plane + n] = S.sf1;
-    if (S.t1 <= 0) {
-      a.s.Fh[3 * plane + n] = 0.0;
-      a.s.Fh[4 * plane + n] = 0.0;
-      a.s.Fh[5 * plane + n] = 0.0;
-    }
+    if (a.s.sff) a.s.sff[plane + n] = (float)S.sf1;
+    else a.s.sf[plane + n] = S.sf1;
+    if (S.t1 <= 0) store_node_vec(a.s.Fh, a.s.Fhf, 3 * plane + n, plane, 0.0, 0.0, 0.0);
   }
   // a slot set this step has its force reset (src/LBM_PSM_lattice.cpp:
   // 257-299): mark it fresh (mask bits 2, 3) and the collide starts its
@@ -751,7 +751,9 @@ constexpr int kHydroThreads = LBMPSM_HYDRO_THREADS;
 constexpr int kHydroU = LBMPSM_HYDRO_U;
 static_assert(kHydroThreads % 64 == 0 && kHydroThreads <= 1024, "whole waves");
 
-__global__ __launch_bounds__(kHydroThreads) void k_hydro(const HydroArgs a) {
+// FT: type of the stored node forces (double for FP64 storage, else float)
+template <typename FT>
+__global__ __launch_bounds__(kHydroThreads) void k_hydro(const HydroArgs a, const FT* Fh) {
   const int p = (int)blockIdx.x / kHydroGroups;
   const int grp = (int)blockIdx.x % kHydroGroups;
   if (p >= a.n) return;
@@ -792,9 +794,8 @@ __global__ __launch_bounds__(kHydroThreads) void k_hydro(const HydroArgs a) {
       for (int u = 0; u < U; ++u) {
         if (sl[u] < 0) continue;
         const int s = sl[u], n = nn[u];
-        const double F0 = a.Fh[(3 * s) * plane + n];
-        const double F1 = a.Fh[(3 * s + 1) * plane + n];
-        const double F2 = a.Fh[(3 * s + 2) * plane + n];
+        const FT* Fs = Fh + (3 * s) * plane + n;
+        const double F0 = Fs[0], F1 = Fs[plane], F2 = Fs[2 * plane];
         acc[0] += F0 * ff;
         acc[1] += F1 * ff;
         acc[2] += F2 * ff;
@@ -862,11 +863,11 @@ __global__ __launch_bounds__(256) void k_slots_export(const Geom g, const SlotsD
   const int64_t P = g.plane;
   for (int c = 0; c < 2; ++c) {
     if (tag) tag[2 * o + c] = s.tag[c * P + n];
-    if (sf) sf[2 * o + c] = s.sf[c * P + n];
+    if (sf) sf[2 * o + c] = s.sff ? (double)s.sff[c * P + n] : s.sf[c * P + n];
   }
   for (int c = 0; c < 6; ++c) {
-    if (us) us[6 * o + c] = s.us[c * P + n];
-    if (F) F[6 * o + c] = s.Fh[c * P + n];
+    if (us) us[6 * o + c] = s.usf ? (double)s.usf[c * P + n] : s.us[c * P + n];
+    if (F) F[6 * o + c] = s.Fhf ? (double)s.Fhf[c * P + n] : s.Fh[c * P + n];
   }
 }
 
@@ -938,7 +939,11 @@ hipError_t launch_map(const MapArgs& a, int64_t nbricks, hipStream_t s) {
 
 hipError_t launch_hydro(const HydroArgs& a, hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_hydro, dim3((unsigned)a.n * kHydroGroups), dim3(kHydroThreads), 0, s, a);
+  const dim3 grid((unsigned)a.n * kHydroGroups);
+  if (a.Fhf)
+    hipLaunchKernelGGL(k_hydro<float>, grid, dim3(kHydroThreads), 0, s, a, a.Fhf);
+  else
+    hipLaunchKernelGGL(k_hydro<double>, grid, dim3(kHydroThreads), 0, s, a, a.Fh);
   hipLaunchKernelGGL(k_hydro_sum, grid_for(12 * (int64_t)a.n, 256), dim3(256), 0, s, a);
   return hipGetLastError();
 }
