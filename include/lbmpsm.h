@@ -179,8 +179,10 @@ int lbmpsm_get_macros(lbmpsm_ctx* ctx, double* rho, double* u);
 /* Per-node particle slots, interior nodes in reference order:
  * tag [node][2], sf [node][2], u_s [node][6], F [node][6] (lattice units)
  * (ParticleDataOnLattice, src/LBM_PSM_particleDataOnLattice.h:22-32);
- * any pointer may be NULL.  FP32 and FP16 storage keep sf, u_s and F in
- * float on the device; they are returned widened to double. */
+ * any pointer may be NULL.  A slot set by the last particle list and not
+ * yet collided reads F = 0, as the reference's set resets it.  FP32 and
+ * FP16 storage keep sf, u_s and F in float on the device; they are returned
+ * widened to double. */
 int lbmpsm_get_slots(lbmpsm_ctx* ctx, int* tag, double* sf, double* us, double* F);
 
 /* Sum of all populations over interior nodes, accumulated in FP64. */

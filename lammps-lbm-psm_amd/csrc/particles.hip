@@ -861,13 +861,19 @@ __global__ __launch_bounds__(256) void k_slots_export(const Geom g, const SlotsD
   int64_t o;
   if (!interior_of(g, n, o)) return;
   const int64_t P = g.plane;
+  // a slot set since the last collide reads zero force, as the reference's
+  // set resets it (src/LBM_PSM_lattice.cpp:257-299); the collide starts such
+  // a force from zero without the map storing one
+  const uint8_t m = s.mask[n];
   for (int c = 0; c < 2; ++c) {
+    const bool fresh = m & (c ? kMaskFresh1 : kMaskFresh0);
     if (tag) tag[2 * o + c] = s.tag[c * P + n];
     if (sf) sf[2 * o + c] = s.sff ? (double)s.sff[c * P + n] : s.sf[c * P + n];
-  }
-  for (int c = 0; c < 6; ++c) {
-    if (us) us[6 * o + c] = s.usf ? (double)s.usf[c * P + n] : s.us[c * P + n];
-    if (F) F[6 * o + c] = s.Fhf ? (double)s.Fhf[c * P + n] : s.Fh[c * P + n];
+    for (int d = 0; d < 3; ++d) {
+      const int64_t v = (3 * c + d) * P + n;
+      if (us) us[6 * o + 3 * c + d] = s.usf ? (double)s.usf[v] : s.us[v];
+      if (F) F[6 * o + 3 * c + d] = fresh ? 0.0 : s.Fhf ? (double)s.Fhf[v] : s.Fh[v];
+    }
   }
 }
 

@@ -167,6 +167,36 @@ def test_particle_leaves_list_and_returns():
     compare(lat, ora, out)
 
 
+def test_slots_read_between_map_and_collide():
+    # the map's slot state is visible before the step: tags, solid fractions
+    # and node velocities as the oracle leaves them (its collide does not
+    # change them), and zero force on every slot the map set, as the
+    # reference's set resets it (src/LBM_PSM_lattice.cpp:257-299)
+    n = (20, 18, 22)
+    per = (1, 1, 1)
+    box = lattice_box(3, n, per)
+    atoms = random_atoms(box, [3.1, 2.7, 3.4, 2.9], seed=23, vmax=2e-2, wmax=2e-3, gap=1.0)
+    lat = lbmpsm.Lattice(lbmpsm.lattice_units_config(3, n, per, 0.8, F_lbm=(2e-6, 0, 0)))
+    ora = O.Oracle(3, n, per, 0.8, F_lbm=(2e-6, 0, 0))
+    print("\n[case] slots after set_particles, before step, 6 steps", flush=True)
+    out = []
+    for step in range(6):
+        H.move_linear(atoms, box, step, 1.0)
+        lst = H.build_list(atoms, box, 7.5)
+        Fo, To, So = ora.fix_step(lst, bc_keyword=0)
+        lat.set_particles(lst.tag, lst.x, lst.v, lst.omega, lst.radius)
+        tg, sfg, usg, Fhg = lat.slots()
+        to, sfo, uso, _ = ora.slots()
+        assert np.array_equal(tg, to), f"step {step}: slot tags differ"
+        assert np.array_equal(sfg, sfo), f"step {step}: solid fractions differ"
+        assert rel_err(usg, uso) <= TOL, f"step {step}: node velocities differ"
+        assert not np.any(Fhg), f"step {step}: a slot set by the map shows a force"
+        lat.step(store_macros=True)
+        Fg, Tg, Sg = lat.hydro()
+        out.append((lst, (Fg, Tg, Sg), (Fo, To, So)))
+    compare(lat, ora, out)
+
+
 @pytest.mark.parametrize("L", [12, 16, 17])
 def test_images_near_their_own_zones(L):
     """A sphere of radius 5 straddling a periodic edge: its images lie a box
