@@ -39,6 +39,14 @@
 #else
 #define LBMPSM_OCC
 #endif
+#ifndef LBMPSM_LOWP_WAVES
+#define LBMPSM_LOWP_WAVES 5  // > 0: waves per SIMD asked of k_collide_lowp
+#endif
+#if LBMPSM_LOWP_WAVES > 0
+#define LBMPSM_LOWP_OCC __attribute__((amdgpu_waves_per_eu(LBMPSM_LOWP_WAVES)))
+#else
+#define LBMPSM_LOWP_OCC
+#endif
 #ifndef LBMPSM_ROW_WAVES
 #define LBMPSM_ROW_WAVES 1  // waves per bulk workgroup; 0: the chunks of one row, at most 4
 #endif
@@ -615,7 +623,7 @@ __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f
                                                   const SlotsF32& S, Emit&& emit);
 
 template <int DIM, int Q, int ST, bool POST>
-__global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_lowp(const CollideArgs a) {
+__global__ __launch_bounds__(256) LBMPSM_LOWP_OCC void k_collide_lowp(const CollideArgs a) {
   const Geom& g = a.g;
   Row r;
   if (!make_row<DIM>(g, bulk_wave(), r)) return;
@@ -708,37 +716,54 @@ __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f
         Btot = 1.0f;
       }
       const float omo = 1.0f - omega;
-      Unroll<0, Q>::run([&]<int q>() {
-        constexpr float w = (float)VelSet<Q>::w[q];
-        constexpr int cx = VelSet<Q>::ex[q], cy = VelSet<Q>::ey[q], cz = VelSet<Q>::ez[q];
-        const float gi = f[q];
-        const float eu = edot<Q, q>(ux, uy, uz);
-        const float geq = w * (drho + rho * (4.5f * eu * eu - usq)) + eq_odd<Q, q>(Ax, Ay, Az);
-        float sc1 = 0.f, sc2 = 0.f;
-        // the solid equilibrium keeps the plain float weight: its momentum
-        // bias, 7.4e-9 rho u_s, is ~1e-8 of the node's exchange B rho (u_s - u)
-        if (o1) {
-          const float e1 = edot<Q, q>(us1[0], us1[1], us1[2]);
-          const float gs = w * (drho + rho * (3.0f * e1 + 4.5f * e1 * e1 - u1sq));
-          sc1 = (gs - gi) + omo * (gi - geq);
-        }
-        if (o2) {
-          const float e2 = edot<Q, q>(us2[0], us2[1], us2[2]);
-          const float gs = w * (drho + rho * (3.0f * e2 + 4.5f * e2 * e2 - u2sq));
-          sc2 = (gs - gi) + omo * (gi - geq);
-        }
-        float out = gi + (1.0f - Btot) * omega * (geq - gi) + B1 * sc1 + B2 * sc2;
-        if (a.guo) {
-          constexpr float ex = cx, ey = cy, ez = cz;
-          const float eF = edot<Q, q>(F0, F1, F2);
-          out += gp * w * (3.0f * ((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) + 9.0f * eu * eF);
-        }
-        emit(std::integral_constant<int, q>{}, out);
-        const float t1 = -B1 * sc1, t2 = -B2 * sc2;
-        if constexpr (cx != 0) { Fa[0] += cx * t1; Fa[3] += cx * t2; }
-        if constexpr (cy != 0) { Fa[1] += cy * t1; Fa[4] += cy * t2; }
-        if constexpr (cz != 0) { Fa[2] += cz * t1; Fa[5] += cz * t2; }
-      });
+      // TWO: some lane of the wave holds a second particle.  Lanes without
+      // one have B2 = 0, so the second-slot terms only add zeros; a wave
+      // without any (all but the few waves over near-contacts) leaves them
+      // out instead of issuing them under an empty lane mask.
+      auto pass = [&]<bool TWO>() {
+        Unroll<0, Q>::run([&]<int q>() {
+          constexpr float w = (float)VelSet<Q>::w[q];
+          constexpr int cx = VelSet<Q>::ex[q], cy = VelSet<Q>::ey[q], cz = VelSet<Q>::ez[q];
+          const float gi = f[q];
+          const float eu = edot<Q, q>(ux, uy, uz);
+          const float geq = w * (drho + rho * (4.5f * eu * eu - usq)) + eq_odd<Q, q>(Ax, Ay, Az);
+          float sc1 = 0.f, sc2 = 0.f;
+          // the solid equilibrium keeps the plain float weight: its momentum
+          // bias, 7.4e-9 rho u_s, is ~1e-8 of the node's exchange B rho (u_s - u)
+          if (o1) {
+            const float e1 = edot<Q, q>(us1[0], us1[1], us1[2]);
+            const float gs = w * (drho + rho * (3.0f * e1 + 4.5f * e1 * e1 - u1sq));
+            sc1 = (gs - gi) + omo * (gi - geq);
+          }
+          if constexpr (TWO) {
+            if (o2) {
+              const float e2 = edot<Q, q>(us2[0], us2[1], us2[2]);
+              const float gs = w * (drho + rho * (3.0f * e2 + 4.5f * e2 * e2 - u2sq));
+              sc2 = (gs - gi) + omo * (gi - geq);
+            }
+          }
+          float out = gi + (1.0f - Btot) * omega * (geq - gi) + B1 * sc1;
+          if constexpr (TWO) out = out + B2 * sc2;
+          if (a.guo) {
+            constexpr float ex = cx, ey = cy, ez = cz;
+            const float eF = edot<Q, q>(F0, F1, F2);
+            out += gp * w * (3.0f * ((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) + 9.0f * eu * eF);
+          }
+          emit(std::integral_constant<int, q>{}, out);
+          const float t1 = -B1 * sc1;
+          if constexpr (cx != 0) Fa[0] += cx * t1;
+          if constexpr (cy != 0) Fa[1] += cy * t1;
+          if constexpr (cz != 0) Fa[2] += cz * t1;
+          if constexpr (TWO) {
+            const float t2 = -B2 * sc2;
+            if constexpr (cx != 0) Fa[3] += cx * t2;
+            if constexpr (cy != 0) Fa[4] += cy * t2;
+            if constexpr (cz != 0) Fa[5] += cz * t2;
+          }
+        });
+      };
+      if (__any(o2)) pass.template operator()<true>();
+      else pass.template operator()<false>();
       // node forces continue from the stored value, or from zero for a slot
       // the map set this step
       if (o1) {
