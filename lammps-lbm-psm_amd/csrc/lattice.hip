@@ -40,10 +40,11 @@
 #define LBMPSM_OCC
 #endif
 #ifndef LBMPSM_LOWP_WAVES
-#define LBMPSM_LOWP_WAVES 5  // > 0: waves per SIMD asked of k_collide_lowp
+#define LBMPSM_LOWP_WAVES 5  // > 0: waves per SIMD asked of the bulk k_collide_lowp
 #endif
 #if LBMPSM_LOWP_WAVES > 0
-#define LBMPSM_LOWP_OCC __attribute__((amdgpu_waves_per_eu(LBMPSM_LOWP_WAVES)))
+// the once-per-run post-stream variant is left to the compiler (it would spill)
+#define LBMPSM_LOWP_OCC __attribute__((amdgpu_waves_per_eu(POST ? 1 : LBMPSM_LOWP_WAVES)))
 #else
 #define LBMPSM_LOWP_OCC
 #endif
