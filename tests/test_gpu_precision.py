@@ -61,3 +61,42 @@ def test_reduced_storage_error(ref64, storage, tol):
           f"mass/node {em:.2e}", flush=True)
     assert eF <= tol and eT <= tol and eu <= tol
     assert em <= (1e-6 if storage == "f32" else 1e-4)
+
+
+def run_contacts(storage, steps=120):
+    """Three pairs of spheres of radius 6 with centres 12.3 apart (along x,
+    along y, diagonally): their rims share nodes, so some waves hold a node
+    with two particles and run the particle form with both slots."""
+    n = (48, 48, 48)
+    per = (1, 1, 1)
+    box = lattice_box(3, n, per)
+    d = 12.3
+    pos = np.array([[12.0, 12.0, 12.0], [12.0 + d, 12.0, 12.0],
+                    [30.0, 28.0, 20.0], [30.0, 28.0 + d, 20.0],
+                    [14.0, 34.0, 36.0], [14.0 + d / np.sqrt(2), 34.0 + d / np.sqrt(2), 36.0]])
+    atoms = random_atoms(box, [6.0] * 6, seed=29, positions=pos, vmax=1e-3, wmax=2e-4)
+    lat = lbmpsm.Lattice(lbmpsm.lattice_units_config(3, n, per, 0.8, storage=storage,
+                                                     F_lbm=(3e-6, 0, 0)))
+    for s in range(steps):
+        H.move_linear(atoms, box, s, 1.0)
+        lst = H.build_list(atoms, box, 10.0)
+        lat.set_particles(lst.tag, lst.x, lst.v, lst.omega, lst.radius)
+        lat.step()
+        F, T, S = lat.hydro()
+    tag, _, _, _ = lat.slots()
+    shared = int(np.count_nonzero((tag[..., 0] > 0) & (tag[..., 1] > 0)))
+    lat.close()
+    return F, T, shared
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f16", 2e-2)])
+def test_reduced_storage_with_shared_nodes(storage, tol):
+    print(f"\n[case] precision {storage} with shared nodes, 48^3, 3 sphere pairs 0.3 apart",
+          flush=True)
+    F0, T0, shared0 = run_contacts("f64")
+    F, T, shared = run_contacts(storage)
+    assert shared0 > 0 and shared == shared0, (shared0, shared)
+    eF = np.max(np.linalg.norm(F - F0, axis=1)) / np.max(np.linalg.norm(F0, axis=1))
+    eT = np.max(np.linalg.norm(T - T0, axis=1)) / np.max(np.linalg.norm(T0, axis=1))
+    print(f"  {storage}: {shared} shared nodes, drag {eF:.2e} torque {eT:.2e}", flush=True)
+    assert eF <= tol and eT <= tol
