@@ -651,6 +651,30 @@ static void zh3_vel_zp(lbo_lattice* L, int iz, int i0, int i1, int j0, int j1, d
 }
 #undef GF
 
+/* Test hook: one face function above on the nodes `plane` x [a0,a1] x
+ * [b0,b1] (reference halo'd indices; the tangential axes in the order the
+ * function takes them), with an arbitrary imposed velocity or density, so
+ * that every formula can be checked against the Zou-He conditions even where
+ * no `lbm-psm-bc` keyword reaches it with u != 0. */
+int lbo_zou_he_face(lbo_lattice* L, int kind, int plane, int a0, int a1, int b0, int b1,
+                    double ux, double uy, double uz, double rho) {
+  switch (kind) {
+    case LBO_ZH2_VEL_XN: zh2_vel_xn(L, plane, a0, a1, ux); return 0;
+    case LBO_ZH2_VEL_XP: zh2_vel_xp(L, plane, a0, a1, ux); return 0;
+    case LBO_ZH2_DENS_XP: zh2_dens_xp(L, plane, a0, a1, rho); return 0;
+    case LBO_ZH2_VEL_YN: zh2_vel_yn(L, plane, a0, a1, ux); return 0;
+    case LBO_ZH2_VEL_YP: zh2_vel_yp(L, plane, a0, a1, ux); return 0;
+    case LBO_ZH3_VEL_XN: zh3_vel_xn(L, plane, a0, a1, b0, b1, ux, uy, uz); return 0;
+    case LBO_ZH3_VEL_XP: zh3_vel_xp(L, plane, a0, a1, b0, b1, ux, uy, uz); return 0;
+    case LBO_ZH3_DENS_XP: zh3_dens_xp(L, plane, a0, a1, b0, b1, rho, uy, uz); return 0;
+    case LBO_ZH3_VEL_YN: zh3_vel_yn(L, plane, a0, a1, b0, b1, ux, uy, uz); return 0;
+    case LBO_ZH3_VEL_YP: zh3_vel_yp(L, plane, a0, a1, b0, b1, ux, uy, uz); return 0;
+    case LBO_ZH3_VEL_ZN: zh3_vel_zn(L, plane, a0, a1, b0, b1, ux, uy, uz); return 0;
+    case LBO_ZH3_VEL_ZP: zh3_vel_zp(L, plane, a0, a1, b0, b1, ux, uy, uz); return 0;
+    default: return -1;
+  }
+}
+
 /* fix_LBM_PSM_BC::post_force, src/fix_LBM_PSM_BC.cpp:151-402, one rank
  * (every myloc test is true).  e = envelopeWidth = 1. */
 void lbo_apply_bc(lbo_lattice* L, int typeBC, double U) {

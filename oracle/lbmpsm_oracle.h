@@ -60,6 +60,19 @@ void lbo_collide_stream(lbo_lattice* L);
  * 4 channel-velIn-pressOut, 5 closedBox. */
 void lbo_apply_bc(lbo_lattice* L, int keyword, double u_infty);
 
+/* One Zou-He face function of src/LBM_PSM_zou_he_BC.cpp on the nodes
+ * plane x [a0,a1] x [b0,b1] (b ignored in 2D) with any imposed velocity
+ * (ux, uy, uz) or density rho (test hook; returns -1 for an unknown kind).
+ * 2D velocity faces take ux only; the 3D density face takes rho, uy, uz. */
+enum {
+  LBO_ZH2_VEL_XN = 0, LBO_ZH2_VEL_XP = 1, LBO_ZH2_DENS_XP = 2, LBO_ZH2_VEL_YN = 3,
+  LBO_ZH2_VEL_YP = 4,
+  LBO_ZH3_VEL_XN = 10, LBO_ZH3_VEL_XP = 11, LBO_ZH3_DENS_XP = 12, LBO_ZH3_VEL_YN = 13,
+  LBO_ZH3_VEL_YP = 14, LBO_ZH3_VEL_ZN = 15, LBO_ZH3_VEL_ZP = 16
+};
+int lbo_zou_he_face(lbo_lattice* L, int kind, int plane, int a0, int a1, int b0, int b1,
+                    double ux, double uy, double uz, double rho);
+
 /* calculateHydrodynamicInteractions for one particle; accumulates into
  * fh[3], th[3], st[6] (the caller zeroes them). */
 void lbo_hydro(const lbo_lattice* L, const lbo_units* U, const double origin[3],

@@ -52,6 +52,8 @@ def _load():
         "lbo_halo_exchange": (None, [P]),
         "lbo_collide_stream": (None, [P]),
         "lbo_apply_bc": (None, [P, C.c_int, C.c_double]),
+        "lbo_zou_he_face": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.c_double, C.c_double, C.c_double, C.c_double]),
         "lbo_hydro": (None, [P, C.POINTER(Units), dp, C.c_int, dp, C.c_double, dp, dp, dp]),
         "lbo_fix_step": (None, [P, C.POINTER(Units), dp, C.c_int, ip, dp, dp, dp, dp, C.c_int,
                                 dp, dp, dp]),
@@ -146,6 +148,14 @@ class Oracle:
                          tag.ctypes.data_as(C.POINTER(C.c_int)), _d(lst.x), _d(lst.v),
                          _d(lst.omega), _d(lst.radius), int(bc_keyword), _d(F), _d(T), _d(S))
         return F, T, S
+
+    def zou_he_face(self, kind, plane, a, b=(0, 0), u=(0.0, 0.0, 0.0), rho=1.0):
+        """One face function on plane x [a0,a1] x [b0,b1] (reference halo'd
+        indices) with the imposed velocity u or density rho."""
+        rc = LIB.lbo_zou_he_face(self._h, int(kind), int(plane), int(a[0]), int(a[1]), int(b[0]),
+                                 int(b[1]), float(u[0]), float(u[1]), float(u[2]), float(rho))
+        if rc:
+            raise ValueError(f"unknown face kind {kind}")
 
     def solid_fraction(self, i, j, k, xp, yp, zp, rp):
         return LIB.lbo_solid_fraction(self._h, i, j, k, xp, yp, zp, rp)
