@@ -1,16 +1,26 @@
 #!/usr/bin/env python3
-"""LBM-PSM throughput on one MI355X: D3Q19 BGK-PSM, 192^3 nodes, shear walls,
-400 moving spheres (d = 16, seeded RSA, ~12 % solids).
+"""LBM-PSM throughput on one MI355X.
 
-One step = one LBM step of `fix lbm-psm`: map the atom list (locals +
-periodic ghosts) onto the lattice, fused collide-stream with PSM weighting
-and the Zou-He shear faces, and the per-particle force/torque/stresslet
-reduction.  The per-step lists are staged in device memory before the timed
-region (`value`); the same steps with a host upload, a result download and
-a synchronisation per step, as the fix runs them, are reported as
-`host_inclusive`.  Prints one JSON line.
+Headline workload: the D3Q19 BGK-PSM fluidised-bed column, 1024 x 256 x 256
+nodes (x vertical, `boundary f p p`), `lbm-psm-bc xFlow` (Zou-He velocity
+inlet U = 0.005 at x = 0, density outlet rho = 1, and velocity faces on the
+y/z planes as the reference's xFlow applies them), gravity Fx = -2e-6,
+tau = 0.7, 20000 spheres d = 12 seeded by RSA over the column (phi = 0.27),
+moving linearly (v ~ U(+-1e-3), omega ~ U(+-1e-4)) so that every step maps
+new positions.  Secondary workload: D3Q19 192^3 shear suspension (y walls at
++-0.01, 400 moving spheres d = 16).
+
+One step = one LBM step of `fix lbm-psm`: map the atom list (locals + ghost
+images) onto the lattice, fused pull-stream + Zou-He + BGK-PSM collide with
+Guo forcing, and the per-particle force/torque/stresslet reduction.  Every
+step's list is staged in device memory before the timed region (`value`,
+FP64 storage); the same steps with a host upload, a result download and a
+synchronisation per step, as the fix runs them, are `host_inclusive`.
+FP32 and FP16 storage are reported under `variants` with their drag and
+torque error against FP64 after the same steps.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--storage f64|f32|f16]
+                  [--workload bed|shear]
 """
 from __future__ import annotations
 
@@ -27,42 +37,47 @@ sys.path.insert(0, os.path.join(ROOT, "lammps-lbm-psm_amd"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 Q = 19
-NX = NY = NZ = 192
-TAU = 0.7            # `fix lbm-psm` default (src/fix_LBM_PSM.cpp:23); the workload names none
-U_WALL = 0.01
-N_SPHERES = 400
-RADIUS = 8.0
-SEED = 20240611
 METRIC = "MLUPS + % HBM roofline, D3Q19 PSM: FP64 vs FP32-storage at stated drag error"
 BYTES_PER_ELEM = {"f64": 8, "f32": 4, "f16": 2}
-
-
-def workload():
-    from lbmpsm import harness as H
-    # LAMMPS `boundary p f p`: 192 periodic nodes in x and z, 192 wall-to-wall
-    # nodes in y (Ly = 191 lattice units)
-    box = H.Box(lo=np.zeros(3), prd=np.array([192.0, 191.0, 192.0]), periodic=(1, 0, 1), dim=3)
-    radii = np.full(N_SPHERES, RADIUS)
-    pos = H.rsa_spheres(box, radii, SEED, gap=2.5)
-    rng = np.random.default_rng(SEED + 1)
-    v = rng.uniform(-1e-3, 1e-3, (N_SPHERES, 3))
-    w = rng.uniform(-1e-4, 1e-4, (N_SPHERES, 3))
-    atoms = H.Atoms(tag=np.arange(1, N_SPHERES + 1, dtype=np.int32), x=pos, v=v, omega=w,
-                    radius=radii)
-    return box, atoms
-
-
-def atom_lists(box, atoms, nsteps):
-    from lbmpsm import harness as H
-    a = atoms.copy()
-    out = []
-    for s in range(nsteps):
-        H.move_linear(a, box, s, 1.0)
-        out.append(H.build_list(a, box, cutoff=RADIUS + 4.0))
-    return out
-
-
 PROF_KERNELS = ("collide_stream", "map_bin", "map_fraction", "map_particles", "hydro_reduce")
+KERNEL_NAME = {"f64": "k_collide_f64<3,19,false>", "f32": "k_collide_lowp<3,19,1,false>",
+               "f16": "k_collide_lowp<3,19,2,false>"}
+
+
+class Workload:
+    def __init__(self, name):
+        from lbmpsm import workloads as W
+        self.name = name
+        if name == "bed":
+            c = W.BED
+            self.box, self.atoms = W.fluidised_bed()
+            self.n, self.per, self.tau, self.F = c["n"], c["periodic"], c["tau"], c["F"]
+            self.u_lb, self.bc, self.radius = c["u_inlet"], c["bc"], c["radius"]
+            self.describe = ("D3Q19 BGK-PSM fluidised-bed column 1024x256x256 (x vertical), "
+                             "xFlow (Zou-He inlet U=0.005, outlet rho=1), Fx=-2e-6, tau=0.7, "
+                             "20000 moving spheres d=12 (seeded RSA, phi=0.27)")
+        else:
+            c = W.SHEAR
+            self.box, self.atoms = W.shear_suspension()
+            self.n, self.per, self.tau, self.F = c["n"], c["periodic"], c["tau"], (0.0, 0.0, 0.0)
+            self.u_lb, self.bc, self.radius = c["u_wall"], c["bc"], c["radius"]
+            self.describe = ("D3Q19 BGK-PSM 192^3, shear (Zou-He walls y, U=0.01), "
+                             "400 moving spheres d=16 (seeded RSA), tau=0.7")
+        self.nodes = self.n[0] * self.n[1] * self.n[2]
+        self.cutoff = self.radius + 4.0
+        self.bc_id = {"shear": 1, "xFlow": 2}[self.bc]
+
+    def lists(self, nsteps):
+        from lbmpsm import workloads as W
+        return W.atom_lists(self.box, self.atoms, nsteps, self.cutoff)
+
+    def lattice(self, storage):
+        import lbmpsm
+        cfg = lbmpsm.lattice_units_config(3, self.n, self.per, self.tau, F_lbm=self.F,
+                                          storage=storage, u_lb=self.u_lb)
+        lat = lbmpsm.Lattice(cfg)
+        lat.set_bc(self.bc)
+        return lat
 
 
 def stage_lists(lists):
@@ -79,15 +94,11 @@ def stage_lists(lists):
     return block, off, rmax
 
 
-def run_gpu(storage, lists, staged, warmup, steps):
+def run_gpu(wl, storage, staged, warmup, steps):
     """Device-resident run: lists already in HBM, map + collide + faces +
     hydro per step with no host transfer inside the timed region."""
-    import lbmpsm
     block, off, rmax = staged
-    cfg = lbmpsm.lattice_units_config(3, (NX, NY, NZ), (1, 0, 1), TAU, storage=storage,
-                                      u_lb=U_WALL)
-    lat = lbmpsm.Lattice(cfg)
-    lat.set_bc("shear")
+    lat = wl.lattice(storage)
     p = {k: t.address for k, t in block.items()}
 
     def one(s):
@@ -99,7 +110,8 @@ def run_gpu(storage, lists, staged, warmup, steps):
     for s in range(warmup):
         one(s)
     lat.sync()
-    # timed region: HIP events around the bulk collide only
+    # timed region: HIP events on the library's stream around the bulk
+    # collide only (the kernel the roofline figure is for)
     lat.profile("collide")
     lat.profile_reset()
     t0 = time.perf_counter()
@@ -112,7 +124,7 @@ def run_gpu(storage, lists, staged, warmup, steps):
     # per-kernel breakdown from a separate, untimed pass over the warm-up lists
     lat.profile(True)
     lat.profile_reset()
-    for s in range(min(warmup, 20)):
+    for s in range(min(warmup, 10)):
         one(s)
     lat.sync()
     prof = {k: lat.profile_read(k) for k in PROF_KERNELS}
@@ -120,17 +132,13 @@ def run_gpu(storage, lists, staged, warmup, steps):
     lat.profile(False)
     info = lat.info
     lat.close()
-    return t1 - t0, F, T, S, prof, info
+    return dict(el=t1 - t0, F=F, T=T, S=S, prof=prof, info=info)
 
 
-def run_gpu_host(storage, lists, warmup, steps):
+def run_gpu_host(wl, storage, lists, warmup, steps):
     """As `fix lbm-psm` drives it: host lists uploaded, results downloaded
     and the stream synchronised every step."""
-    import lbmpsm
-    cfg = lbmpsm.lattice_units_config(3, (NX, NY, NZ), (1, 0, 1), TAU, storage=storage,
-                                      u_lb=U_WALL)
-    lat = lbmpsm.Lattice(cfg)
-    lat.set_bc("shear")
+    lat = wl.lattice(storage)
     for s in range(warmup):
         L = lists[s]
         lat.set_particles(L.tag, L.x, L.v, L.omega, L.radius)
@@ -148,34 +156,114 @@ def run_gpu_host(storage, lists, warmup, steps):
     return t1 - t0
 
 
-def cpu_baseline(box, atoms, nsteps=3):
-    """The CPU oracle (reference algorithm and layout, OpenMP over x) on the
-    same workload: nsteps full LBM steps after one untimed step."""
+def roofline(wl, storage, res):
+    prof, info = res["prof"], res["info"]
+    ms_c, n_c = prof["collide_stream"]
+    bytes_per_update = 2 * Q * BYTES_PER_ELEM[storage]
+    avg_ms = ms_c / max(n_c, 1)
+    # the collide updates every stored node (a wall axis keeps its two halo
+    # planes, whose populations the pull reads and writes too)
+    nodes = int(info.stored_nodes)
+    achieved = bytes_per_update * nodes / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": KERNEL_NAME[storage], "bytes_per_update": bytes_per_update,
+            "nodes_per_launch": nodes, "avg_launch_ms": round(avg_ms, 5)}
+    # HBM bytes per launch of the dominant kernel from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy kernel)
+    tpath = os.path.join(ROOT, "profiles", f"collide_{storage}_{wl.name}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tr = json.load(fh)
+        roof["traffic"] = tr.get("hbm_bytes_per_launch")
+        roof["traffic_source"] = os.path.relpath(tpath, ROOT)
+    return roof
+
+
+def errors_vs_f64(wl, lst, res, ref):
+    """Max over spheres of |X - X_f64| / max |X_f64| for the owner-summed
+    drag and torque (what the fix hands to LAMMPS)."""
+    from lbmpsm import harness as H
+    F64, T64 = H.reduce_to_owners(lst, ref["F"], ref["T"])
+    F, T = H.reduce_to_owners(lst, res["F"], res["T"])
+    eF = float(np.max(np.linalg.norm(F - F64, axis=1)) / np.max(np.linalg.norm(F64, axis=1)))
+    eT = float(np.max(np.linalg.norm(T - T64, axis=1)) / np.max(np.linalg.norm(T64, axis=1)))
+    return eF, eT
+
+
+def cpu_baseline(wl, lists, steps_all=2):
+    """The CPU oracle (reference algorithm and layout, OpenMP over lattice
+    planes and over the nodes of a particle's zone): `steps_all` full steps
+    of the workload on all host threads after one untimed step, and one step
+    on one thread of a 128-plane section of the same column (its spheres)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import lbmpsm_oracle as O
-    lists = atom_lists(box, atoms, nsteps + 1)
-    ora = O.Oracle(3, (NX, NY, NZ), (1, 0, 1), TAU, unit=O.lattice_units(U_WALL))
-    ora.fix_step(lists[0], bc_keyword=1)
+    from lbmpsm import harness as H
+    nthreads = O.threads()
+    ora = O.Oracle(3, wl.n, wl.per, wl.tau, F_lbm=wl.F, unit=O.lattice_units(wl.u_lb))
+    ora.fix_step(lists[0], bc_keyword=wl.bc_id)
     t0 = time.perf_counter()
-    for s in range(1, nsteps + 1):
-        ora.fix_step(lists[s], bc_keyword=1)
+    for s in range(1, steps_all + 1):
+        ora.fix_step(lists[s], bc_keyword=wl.bc_id)
     t1 = time.perf_counter()
     ora.close()
-    mlups = NX * NY * NZ * nsteps / (t1 - t0) / 1e6
-    return {"value": round(mlups, 3), "unit": "MLUPS", "cores": O.threads(), "kind": "port",
-            "sample": f"{nsteps} full steps of the same 192^3 / 400-sphere workload "
-                      f"(after 1 untimed step), FP64, {O.threads()} OpenMP threads"}
+    all_mlups = wl.nodes * steps_all / (t1 - t0) / 1e6
+    # one thread: a section of x-planes [0, 128) of the column with the
+    # spheres whose zones reach into it (wall axis x keeps its faces)
+    sec = 128 if wl.n[0] > 128 else wl.n[0]
+    n1 = (sec, wl.n[1], wl.n[2])
+    box1 = H.Box(lo=np.zeros(3), prd=np.array([sec - 1.0 if not wl.per[0] else float(sec),
+                                              wl.box.prd[1], wl.box.prd[2]]),
+                 periodic=wl.per, dim=3)
+    keep = wl.atoms.x[:, 0] < sec + wl.cutoff
+    a1 = H.Atoms(tag=wl.atoms.tag[keep], x=wl.atoms.x[keep], v=wl.atoms.v[keep],
+                 omega=wl.atoms.omega[keep], radius=wl.atoms.radius[keep])
+    l1 = [H.build_list(a1, box1, wl.cutoff) for _ in range(2)]
+    O.set_threads(1)
+    ora = O.Oracle(3, n1, wl.per, wl.tau, F_lbm=wl.F, unit=O.lattice_units(wl.u_lb))
+    ora.fix_step(l1[0], bc_keyword=wl.bc_id)
+    t0 = time.perf_counter()
+    ora.fix_step(l1[1], bc_keyword=wl.bc_id)
+    t1 = time.perf_counter()
+    ora.close()
+    O.set_threads(nthreads)
+    one_mlups = n1[0] * n1[1] * n1[2] / (t1 - t0) / 1e6
+    return {"value": round(all_mlups, 3), "unit": "MLUPS", "cores": nthreads, "kind": "port",
+            "sample": f"{steps_all} full FP64 steps of the same workload ({wl.nodes} nodes, "
+                      f"{lists[0].n} list entries) after 1 untimed step, {nthreads} OpenMP "
+                      f"threads",
+            "one_core": {"value": round(one_mlups, 3), "unit": "MLUPS", "cores": 1,
+                         "sample": f"1 FP64 step (after 1 untimed) of x-planes [0, {sec}) of "
+                                   f"the same column with its {int(keep.sum())} spheres, "
+                                   f"1 thread"}}
+
+
+def measure(wl, storages, warmup, steps, host=True):
+    lists = wl.lists(warmup + steps)
+    staged = stage_lists(lists)
+    res = {}
+    for st in storages:
+        print(f"[bench] {wl.name} storage {st}: {warmup} warm-up + {steps} timed steps",
+              file=sys.stderr, flush=True)
+        res[st] = run_gpu(wl, st, staged, warmup, steps)
+    host_el = None
+    if host:
+        print(f"[bench] {wl.name} storage {storages[0]}, host lists and per-step download",
+              file=sys.stderr, flush=True)
+        host_el = run_gpu_host(wl, storages[0], lists, warmup, steps)
+    return lists, res, host_el
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--storage", default="f64", choices=["f64", "f32", "f16"])
+    ap.add_argument("--workload", default="bed", choices=["bed", "shear"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-secondary", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -183,72 +271,54 @@ def main():
               "populations that cross each face every step).", file=sys.stderr)
         sys.exit(2)
 
-    box, atoms = workload()
-    lists = atom_lists(box, atoms, args.warmup + args.steps)
-    nodes = NX * NY * NZ
-
-    staged = stage_lists(lists)
-    results = {}
-    order = [args.storage] + ([] if args.no_variants else
-                              [s for s in ("f64", "f32", "f16") if s != args.storage])
-    for st in order:
-        print(f"[bench] storage {st}: {args.warmup} warm-up + {args.steps} timed steps",
-              file=sys.stderr, flush=True)
-        results[st] = run_gpu(st, lists, staged, args.warmup, args.steps)
-    host_el = None
-    if not args.no_variants:
-        print(f"[bench] storage {args.storage}, host lists and per-step download",
-              file=sys.stderr, flush=True)
-        host_el = run_gpu_host(args.storage, lists, args.warmup, args.steps)
-
-    el, F, T, S, prof, info = results[args.storage]
+    wl = Workload(args.workload)
+    storages = [args.storage] + ([] if args.no_variants else
+                                 [s for s in ("f64", "f32", "f16") if s != args.storage])
+    lists, res, host_el = measure(wl, storages, args.warmup, args.steps,
+                                  host=not args.no_variants)
+    main_res = res[args.storage]
+    el = main_res["el"]
     ms_step = el / args.steps * 1e3
-    mlups = nodes * args.steps / el / 1e6
-    # dominant kernel: the collide-stream over every stored node (the shear
-    # faces are applied inside it)
-    bulk_nodes = info.stored_nodes
-    ms_c, n_c = prof["collide_stream"]
-    bytes_per_update = 2 * Q * BYTES_PER_ELEM[args.storage]
-    avg_ms = ms_c / max(n_c, 1)
-    achieved = bytes_per_update * bulk_nodes / (avg_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_collide_f64<3,19,false>" if args.storage == "f64" else
-                      f"k_collide_lowp<3,19,{ {'f32': 1, 'f16': 2}[args.storage]},false>",
-            "bytes_per_update": bytes_per_update, "nodes_per_launch": int(bulk_nodes),
-            "avg_launch_ms": round(avg_ms, 5)}
-    # HBM bytes per launch of the dominant kernel from the committed
-    # rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy kernel)
-    tpath = os.path.join(ROOT, "profiles", f"collide_{args.storage}_192cube_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            tr = json.load(fh)
-        roof["traffic"] = tr.get("hbm_bytes_per_launch")
-        roof["traffic_source"] = os.path.relpath(tpath, ROOT)
+    mlups = wl.nodes * args.steps / el / 1e6
+    roof = roofline(wl, args.storage, main_res)
     from lbmpsm import probe
-    copy_gbs, _ = probe.copy_bandwidth()
+    print("[bench] copy ceiling", file=sys.stderr, flush=True)
+    copy_gbs, copy_form, copy_all = probe.copy_ceiling()
     roof["copy_ceiling"] = round(copy_gbs, 1)
-    roof["frac_of_copy"] = round(achieved / copy_gbs, 4)
+    roof["copy_form"] = probe.FORMS[copy_form]
+    roof["copy_forms"] = {probe.FORMS[f]: round(v, 1) for f, v in copy_all.items()}
+    roof["frac_of_copy"] = round(roof["achieved"] / copy_gbs, 4)
     kernels = {k: {"avg_ms": round(v[0] / max(v[1], 1), 5), "launches": v[1]}
-               for k, v in prof.items()}
-
+               for k, v in main_res["prof"].items()}
     variants = {}
-    Fref = results["f64"][1] if "f64" in results else None
-    for st, (el2, F2, T2, S2, prof2, _) in results.items():
-        d = {"MLUPS": round(nodes * args.steps / el2 / 1e6, 1),
-             "ms_per_step": round(el2 / args.steps * 1e3, 4)}
-        if Fref is not None and st != "f64":
-            scale = np.max(np.linalg.norm(Fref, axis=1))
-            d["drag_rel_err_vs_f64"] = float(np.max(np.linalg.norm(F2 - Fref, axis=1)) / scale)
-            tscale = np.max(np.linalg.norm(results["f64"][2], axis=1))
-            d["torque_rel_err_vs_f64"] = float(np.max(np.linalg.norm(T2 - results["f64"][2],
-                                                                     axis=1)) / tscale)
+    for st, r in res.items():
+        d = {"MLUPS": round(wl.nodes * args.steps / r["el"] / 1e6, 1),
+             "ms_per_step": round(r["el"] / args.steps * 1e3, 4),
+             "collide_ms": round(r["prof"]["collide_stream"][0] /
+                                 max(r["prof"]["collide_stream"][1], 1), 5),
+             "collide_frac_of_peak": roofline(wl, st, r)["frac"]}
+        if "f64" in res and st != "f64":
+            eF, eT = errors_vs_f64(wl, lists[-1], r, res["f64"])
+            d["drag_rel_err_vs_f64"] = eF
+            d["torque_rel_err_vs_f64"] = eT
         variants[st] = d
+    if "f64" in res and "f32" in res:
+        variants["f32_over_f64"] = round(res["f64"]["el"] / res["f32"]["el"], 3)
+
+    secondary = None
+    if not args.no_secondary:
+        other = Workload("shear" if wl.name == "bed" else "bed")
+        _, res2, _ = measure(other, ["f64", "f32"], args.warmup, args.steps, host=False)
+        secondary = {"workload": other.describe}
+        for st, r in res2.items():
+            secondary[st] = {"MLUPS": round(other.nodes * args.steps / r["el"] / 1e6, 1),
+                             "ms_per_step": round(r["el"] / args.steps * 1e3, 4),
+                             "roofline": roofline(other, st, r)}
 
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(box, atoms, args.cpu_steps)
+        cpu = cpu_baseline(wl, lists)
 
     line = {
         "metric": METRIC,
@@ -263,17 +333,16 @@ def main():
         "vs_baseline": None,
         "dtype": args.storage,
         "data": "synthetic",
-        "config": {"workload": "D3Q19 BGK-PSM 192^3, shear (Zou-He walls y, U=0.01), "
-                               "400 moving spheres d=16 (seeded RSA), tau=0.7",
-                   "nodes": nodes, "spheres": N_SPHERES, "list_entries": int(lists[-1].n),
-                   "storage": args.storage},
+        "config": {"workload": wl.describe, "nodes": wl.nodes, "spheres": int(wl.atoms.n),
+                   "list_entries": int(lists[-1].n), "storage": args.storage},
         "roofline": roof,
         "kernels": kernels,
         "variants": variants,
         "host_inclusive": None if host_el is None else {
-            "MLUPS": round(nodes * args.steps / host_el / 1e6, 1),
+            "MLUPS": round(wl.nodes * args.steps / host_el / 1e6, 1),
             "ms_per_step": round(host_el / args.steps * 1e3, 4),
             "note": "per step: host list upload, map, step, F/T/S download, sync (as the fix)"},
+        "secondary": secondary,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -98,6 +98,8 @@ struct lbmpsm_ctx {
     double* bar = nullptr;          // [n][12]
     double* eout = nullptr;         // [n][6]
     double* partials = nullptr;     // [1024][4]
+    double* Fh_save = nullptr;      // [6][plane] slot forces of the forward's last step
+    uint8_t* mask_save = nullptr;   // [plane] slot masks of the forward's last step
     size_t ckpt_n = 0, seg_n = 0, ent_n = 0;
   } adj;
 
@@ -751,7 +753,8 @@ int lbmpsm_destroy(lbmpsm_ctx* c) {
   dev_free(c->d_faces);
   {
     auto& A = c->adj;
-    void* aa[] = {A.ckpt, A.seg, A.lam, A.acc, A.seed, A.writer, A.bar, A.eout, A.partials};
+    void* aa[] = {A.ckpt, A.seg, A.lam, A.acc, A.seed, A.writer, A.bar, A.eout, A.partials,
+                  A.Fh_save, A.mask_save};
     for (void* p : aa) dev_free(p);
   }
   if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -1183,6 +1186,8 @@ int lbmpsm_adjoint_backward(lbmpsm_ctx* c, int n, const double* Fbar, const doub
     rc |= dev_alloc(c, &A.seed, 6 * P);
     rc |= dev_alloc(c, &A.writer, 2 * P);
     rc |= dev_alloc(c, &A.partials, 1024 * 4);
+    rc |= dev_alloc(c, &A.Fh_save, 6 * P);
+    rc |= dev_alloc(c, &A.mask_save, P);
     if (rc) return LBMPSM_ERR_NOMEM;
   }
   if (A.ent_n < (size_t)std::max(n, 1)) {  // per-entry buffers grow with n
@@ -1228,6 +1233,13 @@ int lbmpsm_adjoint_backward(lbmpsm_ctx* c, int n, const double* Fbar, const doub
   HIPCHK(c, launch_adj_writer(sa, c->stream));
   HIPCHK(c, hipMemsetAsync(A.acc, 0, 10 * P * sizeof(double), c->stream));
 
+  // The recomputing collides below accumulate node forces into the slots and
+  // clear fresh bits, as any collide does; the forward's final slot forces and
+  // masks are saved here and put back at the end, so that the context leaves
+  // backward in the state the forward left it.
+  HIPCHK(c, hipMemcpyAsync(A.Fh_save, c->slots.Fh, 6 * P * sizeof(double),
+                           hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(A.mask_save, c->slots.mask, P, hipMemcpyDeviceToDevice, c->stream));
   CollideArgs base = make_args(c, 0);
   AdjArgs aa{};
   aa.g = g;
@@ -1269,6 +1281,9 @@ int lbmpsm_adjoint_backward(lbmpsm_ctx* c, int n, const double* Fbar, const doub
       have_lam = true;
     }
   }
+  HIPCHK(c, hipMemcpyAsync(c->slots.Fh, A.Fh_save, 6 * P * sizeof(double),
+                           hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->slots.mask, A.mask_save, P, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, launch_adj_entry(sa, A.acc, A.eout, c->stream));
   const int nb = (int)std::min<int64_t>(1024, (g.nodes + 255) / 256);
   HIPCHK(c, launch_adj_param(g, A.acc, A.partials, nb, c->stream));

@@ -579,6 +579,9 @@ __device__ __forceinline__ void map_brick(const MapArgs& a, int64_t b, MapEntry*
   NodeSlots S;
   S.t0 = valid ? a.s.tag[n] : 0;
   S.t1 = valid ? a.s.tag[plane + n] : 0;
+  // the fresh bits of an earlier map whose step has not run yet stay with
+  // the slots this map leaves alone
+  const uint8_t m_old = valid ? a.s.mask[n] : (uint8_t)0;
   S.d0 = S.d1 = false;
   S.sf0 = S.sf1 = 0.0;
   // reference (halo'd) node indices, as used in the geometry
@@ -699,11 +702,14 @@ __device__ __forceinline__ void map_brick(const MapArgs& a, int64_t b, MapEntry*
   }
   // a slot set this step has its force reset (src/LBM_PSM_lattice.cpp:
   // 257-299): mark it fresh (mask bits 2, 3) and the collide starts its
-  // force from zero instead of reading it
-  if (S.d0 || S.d1)
-    a.s.mask[n] = (uint8_t)((S.t0 > 0 ? kMaskOcc0 : 0) | (S.t1 > 0 ? kMaskOcc1 : 0) |
-                            (S.d0 && S.t0 > 0 ? kMaskFresh0 : 0) |
-                            (S.d1 && S.t1 > 0 ? kMaskFresh1 : 0));
+  // force from zero instead of reading it.  A slot this map did not write
+  // keeps its fresh bit: it was set by an earlier map since the last step
+  // (the caller set two lists in a row), so its force is still zero.
+  if (S.d0 || S.d1) {
+    const uint8_t f0 = S.d0 ? (S.t0 > 0 ? kMaskFresh0 : 0) : (m_old & kMaskFresh0);
+    const uint8_t f1 = S.d1 ? (S.t1 > 0 ? kMaskFresh1 : 0) : (m_old & kMaskFresh1);
+    a.s.mask[n] = (uint8_t)((S.t0 > 0 ? kMaskOcc0 : 0) | (S.t1 > 0 ? kMaskOcc1 : 0) | f0 | f1);
+  }
 }
 
 // The common case: bricks whose entries fit their list and all have codes.

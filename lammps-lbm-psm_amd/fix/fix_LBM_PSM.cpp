@@ -172,7 +172,11 @@ void fix_LBM_PSM::init() {
                       lbmpsm_last_error(nullptr);
     error->all(FLERR, msg);
   }
-  if (bc_keyword != LBMPSM_BC_NONE) set_boundary(bc_keyword);
+  // no faces until a `fix lbm-psm-bc` sets them: Modify::init calls its
+  // init() after this one (it must be defined after `fix lbm-psm`), and a BC
+  // fix that was unfixed sets none, so the new lattice runs without faces,
+  // as the reference does once its post_force no longer applies them
+  bc_keyword = LBMPSM_BC_NONE;
 
   for (int i = 0; i < atom->nmax; ++i)
     for (int c2 = 0; c2 < kHydroCols; ++c2) hydrodynamicInteractions[i][c2] = 0.0;

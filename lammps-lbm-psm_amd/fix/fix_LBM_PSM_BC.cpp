@@ -55,7 +55,14 @@ int fix_LBM_PSM_BC::setmask() {
 }
 
 void fix_LBM_PSM_BC::init() {
-  // `fix lbm-psm` (defined earlier) has built its lattice in its own init
+  // looked up again: the `fix lbm-psm` found at construction may have been
+  // unfixed since.  It (defined earlier) has built its lattice in its own
+  // init, which leaves the lattice without faces until this call.
+  fixLBMPSM = nullptr;
+  for (int ifix = 0; ifix < modify->nfix; ifix++)
+    if (std::strcmp(modify->fix[ifix]->style, "lbm-psm") == 0)
+      fixLBMPSM = static_cast<fix_LBM_PSM*>(modify->fix[ifix]);
+  if (!fixLBMPSM) error->all(FLERR, "fix lbm-psm-bc requires a fix lbm-psm defined before it");
   fixLBMPSM->set_boundary(typeBC);
 }
 

@@ -6,6 +6,7 @@
 // hands them over), clears f and torque, calls pre_force and then
 // post_force of every fix in order and lets reverse_comm add each ghost's
 // force and torque to its owner.  Test infrastructure only.
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <sstream>
@@ -141,6 +142,22 @@ int mock_fix(mock_lmp* m, const char* line) {
     if (m->lmp.atom->callback_requested) m->callbacks.push_back(f.get());
     m->lmp.modify->add(f.get());
     m->fixes.push_back(std::move(f));
+  });
+}
+
+// `unfix ID`: the fix is deleted and leaves the fix list
+int mock_unfix(mock_lmp* m, const char* id) {
+  return guarded(m, [&] {
+    for (size_t i = 0; i < m->fixes.size(); ++i)
+      if (std::strcmp(m->fixes[i]->id, id) == 0) {
+        Fix* f = m->fixes[i].get();
+        m->lmp.modify->remove(f);
+        m->callbacks.erase(std::remove(m->callbacks.begin(), m->callbacks.end(), f),
+                           m->callbacks.end());
+        m->fixes.erase(m->fixes.begin() + (long)i);
+        return;
+      }
+    throw std::runtime_error(std::string("Could not find fix ID ") + id + " to delete");
   });
 }
 

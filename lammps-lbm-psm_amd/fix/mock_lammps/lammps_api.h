@@ -168,6 +168,15 @@ class Modify {
     fix = list.data();
     nfix = (int)list.size();
   }
+  void remove(Fix* f) {
+    for (size_t i = 0; i < list.size(); ++i)
+      if (list[i] == f) {
+        list.erase(list.begin() + (long)i);
+        break;
+      }
+    fix = list.data();
+    nfix = (int)list.size();
+  }
   // fixes whose style matches a pattern; only the "^name" prefix form
   std::vector<Fix*> get_fix_by_style(const std::string& pattern) const;
 };

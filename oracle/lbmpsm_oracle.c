@@ -141,6 +141,14 @@ int lbo_threads(void) {
 #endif
 }
 
+void lbo_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* feq from the node's stored rho and u, src/LBM_PSM_dynamics.h:43-59 */
 static inline double feq_node(const lbo_lattice* L, int iq, int n1) {
   const double ex_i = L->ex[iq];
@@ -313,6 +321,12 @@ void lbo_set_particles(lbo_lattice* L, const lbo_units* U, const double origin[3
     const double omegaLB[3] = {omega[3 * p + 0] * U->tc * U->dt_d,
                                omega[3 * p + 1] * U->tc * U->dt_d,
                                omega[3 * p + 2] * U->tc * U->dt_d};
+    /* The nodes of one particle's zone are independent of each other (each
+     * reads and writes only its own record), so they are split over threads;
+     * particles stay in list order, which is all a node's state depends on. */
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2) schedule(static) if ((zone[0][1] - zone[0][0]) * (zone[1][1] - zone[1][0]) * (zone[2][1] - zone[2][0]) > 4096)
+#endif
     for (int i = zone[0][0]; i < zone[0][1]; ++i)
       for (int j = zone[1][0]; j < zone[1][1]; ++j)
         for (int k = zone[2][0]; k < zone[2][1]; ++k) {

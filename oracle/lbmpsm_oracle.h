@@ -100,6 +100,8 @@ double lbo_mass(const lbo_lattice* L);
 
 /* Number of OpenMP threads the collide loop uses (1 when built without). */
 int lbo_threads(void);
+/* Threads of the later OpenMP regions (n > 0). */
+void lbo_set_threads(int n);
 
 #ifdef __cplusplus
 }

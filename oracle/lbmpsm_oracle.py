@@ -66,6 +66,7 @@ def _load():
         "lbo_slot_F": (dp, [P]),
         "lbo_mass": (C.c_double, [P]),
         "lbo_threads": (C.c_int, []),
+        "lbo_set_threads": (None, [C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -149,6 +150,13 @@ class Oracle:
                          _d(lst.omega), _d(lst.radius), int(bc_keyword), _d(F), _d(T), _d(S))
         return F, T, S
 
+    def map(self, lst):
+        """Only the particle map of a step (setParticlesOnLattice)."""
+        tag = np.ascontiguousarray(lst.tag, dtype=np.int32)
+        LIB.lbo_set_particles(self._h, C.byref(self.unit), _d(self.origin), lst.n,
+                              tag.ctypes.data_as(C.POINTER(C.c_int)), _d(lst.x), _d(lst.v),
+                              _d(lst.omega), _d(lst.radius))
+
     def zou_he_face(self, kind, plane, a, b=(0, 0), u=(0.0, 0.0, 0.0), rho=1.0):
         """One face function on plane x [a0,a1] x [b0,b1] (reference halo'd
         indices) with the imposed velocity u or density rho."""
@@ -203,3 +211,7 @@ class Oracle:
 
 def threads():
     return LIB.lbo_threads()
+
+
+def set_threads(n):
+    LIB.lbo_set_threads(int(n))

@@ -40,6 +40,8 @@ def _lib():
     lib.mock_ntimestep.argtypes = [C.c_void_p]
     lib.mock_fix.restype = C.c_int
     lib.mock_fix.argtypes = [C.c_void_p, C.c_char_p]
+    lib.mock_unfix.restype = C.c_int
+    lib.mock_unfix.argtypes = [C.c_void_p, C.c_char_p]
     lib.mock_init.restype = C.c_int
     lib.mock_init.argtypes = [C.c_void_p]
     lib.mock_step.restype = C.c_int
@@ -100,6 +102,10 @@ class Lammps:
     def fix(self, line):
         """`fix ID group style args...` (without the word `fix`)."""
         self._ok(self._lib.mock_fix(self._h, line.encode()))
+
+    def unfix(self, fix_id):
+        """`unfix ID`."""
+        self._ok(self._lib.mock_unfix(self._h, fix_id.encode()))
 
     def init(self):
         self._ok(self._lib.mock_init(self._h))

@@ -21,6 +21,7 @@ import lbmpsm
 import lbmpsm_oracle as O
 from lammps_mock import Lammps, LammpsError
 from lbmpsm import harness as H
+from scenarios import check_hydro, hydro_scales, owner_scales
 
 pytestmark = pytest.mark.skipif(
     not os.path.exists(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
@@ -115,6 +116,8 @@ def test_fix_deck_against_oracle():
     stored_F = np.zeros((atoms.n, 3))
     stored_T = np.zeros((atoms.n, 3))
     vir_ref = np.zeros(6)
+    oF = oT = np.zeros(atoms.n)   # bounds of the sums of |terms| (none before the first LBM step)
+    vs = np.zeros(1)
     print(f"\n[case] fix deck 32x33x32 every 2, 12 steps", flush=True)
     for step in range(1, 13):
         H.move_linear(atoms, box, step, dt)
@@ -126,13 +129,55 @@ def test_fix_deck_against_oracle():
             Fo, To = H.reduce_to_owners(lst, F, T)
             stored_F, stored_T = F[:lst.nlocal].copy(), T[:lst.nlocal].copy()
             vir_ref = S.sum(axis=0)
+            u = ora.unit
+            to, _, _, Fho = ora.slots()
+            sF, sT = hydro_scales(lst, to, Fho, u.forceFactor, u.torqueFactor, u.dx)
+            oF, oT = owner_scales(lst, sF, sT)
+            vs = np.array([sT.sum()])
         else:
             Fo, To = stored_F, stored_T
-        sF = max(np.max(np.abs(Fo)), 1e-300)
-        sT = max(np.max(np.abs(To)), 1e-300)
-        assert np.max(np.abs(f - Fo)) <= 1e-10 * sF, step
-        assert np.max(np.abs(t - To)) <= 1e-10 * sT, step
-        assert np.max(np.abs(vir - vir_ref)) <= 1e-10 * max(np.max(np.abs(vir_ref)), 1e-300)
+        check_hydro(f, Fo, oF, 1e-12, f"step {step} f")
+        check_hydro(t, To, oT, 1e-12, f"step {step} torque")
+        check_hydro(vir[None, :], vir_ref[None, :], vs, 1e-12, f"step {step} virial")
     assert np.max(np.abs(f)) > 0.0
+    ora.close()
+    lmp.close()
+
+
+@pytest.mark.gpu
+def test_unfixed_bc_stops_applying_faces():
+    """`unfix` of `fix lbm-psm-bc` between two runs: the second run (a new
+    lattice, as every init builds one) has no Zou-He faces, as the reference
+    stops applying them once its BC fix is gone."""
+    box = _box()
+    lmp = Lammps(box)
+    lmp.fix(FIX)
+    lmp.fix("2 all lbm-psm-bc shear")
+    lmp.init()
+    dt = lmp.dt
+    atoms = _deck_atoms()
+    print("\n[case] fix deck, shear run then unfix lbm-psm-bc and a second run", flush=True)
+    for step in range(1, 5):
+        H.move_linear(atoms, box, step, dt)
+        lmp.step(H.build_list(atoms, box, cutoff=0.0004))
+    lmp.unfix("2")
+    lmp.init()
+    cfg = lbmpsm.config_from_fix(3, NLC, LC, RHO, NU, RE, TAU, (0, 0, 0), box.lo, box.prd,
+                                 box.periodic)
+    ora = O.Oracle(3, (cfg.nx, cfg.ny, cfg.nz), box.periodic, TAU,
+                   unit=O.units(RHO, NU, LC, RE, NLC, TAU, 3), origin=box.lo)
+    for step in range(5, 11):
+        H.move_linear(atoms, box, step, dt)
+        lst = H.build_list(atoms, box, cutoff=0.0004)
+        f, t, vir = lmp.step(lst)
+        if step % 2 == 0:
+            F, T, S = ora.fix_step(lst, bc_keyword=0)
+            Fo, To = H.reduce_to_owners(lst, F, T)
+            u = ora.unit
+            to, _, _, Fho = ora.slots()
+            oF, oT = owner_scales(lst, *hydro_scales(lst, to, Fho, u.forceFactor,
+                                                     u.torqueFactor, u.dx))
+            check_hydro(f, Fo, oF, 1e-12, f"step {step} f")
+            check_hydro(t, To, oT, 1e-12, f"step {step} torque")
     ora.close()
     lmp.close()

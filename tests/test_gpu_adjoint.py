@@ -166,3 +166,24 @@ def test_adjoint_27_spheres_96cube():
     ntheta = 3 + 6 * 27 + 1
     comps = sorted(rng.choice(ntheta, size=20, replace=False).tolist())
     check(prob, F_lbm, v, w, 0.7, comps, "3D cfg")
+
+
+def test_backward_leaves_the_forward_state():
+    """The recomputation inside backward runs collides of its own; the slot
+    forces, slots and populations the forward left stay as they were."""
+    print("\n[case] adjoint leaves forward state, D3Q19 24^3, 2 spheres, 12 steps", flush=True)
+    lat = lbmpsm.Lattice(lbmpsm.lattice_units_config(3, (24, 24, 24), (1, 1, 1), 0.75,
+                                                     F_lbm=(2e-5, 0.0, 0.0)))
+    one = dict(tag=np.array([1, 2], np.int32), x=np.array([[8.3, 11.6, 12.2], [15.1, 12.4, 11.0]]),
+               v=np.array([[1e-3, 0.0, 0.0], [0.0, -1e-3, 0.0]]), omega=np.zeros((2, 3)),
+               radius=np.array([3.5, 3.2]))
+    lat.adjoint_forward(12, 5, **one)
+    t0, sf0, us0, F0 = lat.slots()
+    f0 = lat.populations()
+    lat.adjoint_backward(np.ones((2, 3)))
+    t1, sf1, us1, F1 = lat.slots()
+    assert np.any(F0 != 0.0)
+    assert np.array_equal(t0, t1) and np.array_equal(sf0, sf1) and np.array_equal(us0, us1)
+    assert np.array_equal(F0, F1), "backward changed the slot forces"
+    assert np.array_equal(f0, lat.populations())
+    lat.close()
