@@ -145,6 +145,23 @@ int lbmpsm_set_particles_device(lbmpsm_ctx* ctx, int n, const int* tag, const do
  * calculateHydrodynamicInteractions loop), and fix_LBM_PSM_BC::post_force. */
 int lbmpsm_step(lbmpsm_ctx* ctx, int flags);
 
+/* lbmpsm_step without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid it).  No counterpart in the reference.  The step is the same
+ * (same kernels, same results); the group sums of the force/torque/stresslet
+ * reduction go to a static device array of the library (16 MiB per device,
+ * held until the process ends) instead of context memory, and the per-entry
+ * results go to `out`, caller device memory of [n][12] doubles per list
+ * entry: F (3), T (3), S (6: xx yy zz xy xz yz), physical units, n the
+ * length of the list last given to lbmpsm_set_particles(_device).
+ * Limits and errors: at most 43690 list entries (LBMPSM_ERR_LIMIT beyond);
+ * LBMPSM_STEP_STORE_MACROS only after an lbmpsm_step with that flag has
+ * allocated the macro arrays (LBMPSM_ERR_STATE otherwise); no profiling.
+ * The static array is shared by every call on one device: calls that
+ * overlap on one GPU (two contexts on two streams) race.  lbmpsm_get_hydro
+ * reports LBMPSM_ERR_STATE after this call until the next lbmpsm_step. */
+int lbmpsm_step_noalloc(lbmpsm_ctx* ctx, int flags, double* out);
+
 /* Per-list-entry hydrodynamic force [n][3], torque [n][3] and stresslet
  * [n][6] (xx yy zz xy xz yz) of the last step, physical units; any pointer
  * may be NULL.  Replaces the fh/th/stresslet outputs of

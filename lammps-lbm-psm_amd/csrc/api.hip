@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -525,6 +526,77 @@ int next_list(lbmpsm_ctx* c) {
   return LBMPSM_OK;
 }
 
+// Static scratch of lbmpsm_step_noalloc: the per-entry group sums of the
+// force/torque/stresslet reduction ([n][kHydroGroups][12] doubles) for
+// callers that cannot allocate.  Each device holds its own copy at its own
+// address; host code uses that address (noalloc_scratch), never the name.
+constexpr int64_t kNoAllocScratchDoubles = (int64_t)1 << 21;  // 16 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+constexpr int kMaxDevices = 64;
+double* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+// Once per device, with that device current: the device address of its
+// copy of `scratch`.
+int noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return LBMPSM_ERR_LIMIT;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device]) return LBMPSM_OK;
+  double* ptr = nullptr;
+  if (hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+    return LBMPSM_ERR_HIP;
+  g_scratch_ptr[device] = ptr;
+  return LBMPSM_OK;
+}
+
+double* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  return g_scratch_ptr[device];
+}
+
+// The launches of one step: collide-stream (faces included) and the
+// force/torque/stresslet reduction with its group sums in `part` and its
+// per-entry results in `out` ([n][12], device memory).
+int step_launch(lbmpsm_ctx* c, int store, double* part, double* out, bool own_out) {
+  // the allocation-free form records no profiling events
+  struct ProfOff {
+    lbmpsm_ctx* c;
+    int saved;
+    ~ProfOff() { c->prof = saved; }
+  } prof_off{c, c->prof};
+  if (!own_out) c->prof = 0;
+  CollideArgs a = make_args(c, store);
+  {
+    ProfScope ps(c, "collide_stream");
+    HIPCHK(c, launch_collide(c->cfg.storage, a, c->stream));
+  }
+  c->cur = 1 - c->cur;
+  c->poststream = 0;
+  if (store) c->have_macros = 1;
+  HydroArgs h{};
+  h.g = c->g;
+  h.n = c->part.n;
+  h.tag = c->part.tag(c->part.n);
+  h.xl = c->part.xl;
+  h.rzone = c->part.rzone;
+  h.slot_tag = c->slots.tag;
+  h.Fh = c->slots.Fh;
+  h.Fhf = c->slots.Fhf;
+  h.forceFactor = c->cfg.force_factor;
+  h.torqueFactor = c->cfg.torque_factor;
+  h.part = part;
+  h.out = out;
+  if (h.n > 0) {
+    ProfScope ps(c, "hydro_reduce");
+    HIPCHK(c, launch_hydro(h, c->stream));
+  }
+  // lbmpsm_get_hydro reads the context's own results only
+  c->hydro_ready = own_out ? 1 : 0;
+  c->hydro_n = c->part.n;
+  return LBMPSM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -622,6 +694,10 @@ int lbmpsm_create(const lbmpsm_config* cfg, lbmpsm_ctx** out) {
   if (hipSetDevice(c->device) != hipSuccess) {
     delete c;
     return fail(nullptr, LBMPSM_ERR_HIP, "hipSetDevice failed");
+  }
+  if (noalloc_setup(c->device) != LBMPSM_OK) {
+    delete c;
+    return fail(nullptr, LBMPSM_ERR_HIP, "hipGetSymbolAddress of the static scratch failed");
   }
   Geom& g = c->g;
   g.dim = cfg->dim;
@@ -888,34 +964,26 @@ int lbmpsm_step(lbmpsm_ctx* c, int flags) {
     rc |= dev_alloc(c, &c->u, 3 * (size_t)c->g.plane);
     if (rc) return LBMPSM_ERR_NOMEM;
   }
-  CollideArgs a = make_args(c, store);
-  {
-    ProfScope ps(c, "collide_stream");
-    HIPCHK(c, launch_collide(c->cfg.storage, a, c->stream));
-  }
-  c->cur = 1 - c->cur;
-  c->poststream = 0;
-  if (store) c->have_macros = 1;
-  HydroArgs h{};
-  h.g = c->g;
-  h.n = c->part.n;
-  h.tag = c->part.tag(c->part.n);
-  h.xl = c->part.xl;
-  h.rzone = c->part.rzone;
-  h.slot_tag = c->slots.tag;
-  h.Fh = c->slots.Fh;
-  h.Fhf = c->slots.Fhf;
-  h.forceFactor = c->cfg.force_factor;
-  h.torqueFactor = c->cfg.torque_factor;
-  h.part = c->part.hpart;
-  h.out = c->part.out;
-  if (h.n > 0) {
-    ProfScope ps(c, "hydro_reduce");
-    HIPCHK(c, launch_hydro(h, c->stream));
-  }
-  c->hydro_ready = 1;
-  c->hydro_n = c->part.n;
-  return LBMPSM_OK;
+  return step_launch(c, store, c->part.hpart, c->part.out, true);
+}
+
+int lbmpsm_step_noalloc(lbmpsm_ctx* c, int flags, double* out) {
+  if (!c) return LBMPSM_ERR_ARG;
+  const int n = c->part.n;
+  if (n > 0 && !out) return fail(c, LBMPSM_ERR_ARG, "null output array");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int store = (flags & LBMPSM_STEP_STORE_MACROS) ? 1 : 0;
+  if (store && !c->rho)
+    return fail(c, LBMPSM_ERR_STATE,
+                "LBMPSM_STEP_STORE_MACROS needs the macro arrays of an earlier lbmpsm_step "
+                "with that flag (this entry point allocates nothing)");
+  if ((int64_t)n * kHydroGroups * 12 > kNoAllocScratchDoubles)
+    return fail(c, LBMPSM_ERR_LIMIT,
+                "particle list longer than the static scratch holds (" +
+                    std::to_string(kNoAllocScratchDoubles / (kHydroGroups * 12)) + " entries)");
+  double* scr = noalloc_scratch(c->device);
+  if (!scr) return fail(c, LBMPSM_ERR_STATE, "static scratch address not set up on this device");
+  return step_launch(c, store, scr, out, false);
 }
 
 int lbmpsm_get_hydro(lbmpsm_ctx* c, int n, double* F, double* T, double* S) {

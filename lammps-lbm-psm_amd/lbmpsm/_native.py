@@ -107,6 +107,7 @@ def _load() -> C.CDLL:
         "lbmpsm_set_particles_device": (C.c_int, [P, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                                   C.c_void_p, C.c_void_p, C.c_double]),
         "lbmpsm_step": (C.c_int, [P, C.c_int]),
+        "lbmpsm_step_noalloc": (C.c_int, [P, C.c_int, P]),
         "lbmpsm_get_hydro": (C.c_int, [P, C.c_int, dp, dp, dp]),
         "lbmpsm_get_populations": (C.c_int, [P, dp]),
         "lbmpsm_set_populations": (C.c_int, [P, dp]),

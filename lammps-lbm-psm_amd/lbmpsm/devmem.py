@@ -64,6 +64,16 @@ class DeviceArray:
     def address(self) -> int:
         return int(self.ptr.value or 0)
 
+    def to_host(self) -> np.ndarray:
+        """A host copy (the device is synchronised first)."""
+        hip = _hip()
+        out = np.empty(self.shape, dtype=self.dtype)
+        if hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
+        if self.nbytes and hip.hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2) != 0:
+            raise RuntimeError("hipMemcpy device->host failed")
+        return out
+
     def close(self):
         if self.ptr and self.ptr.value:
             _hip().hipDeviceSynchronize()
