@@ -42,6 +42,7 @@ struct lbmpsm_ctx {
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_sf = nullptr;
   hipEvent_t ev_mapdone = nullptr;
+  hipEvent_t ev_collide = nullptr;  // end of the last collide (LBMPSM_SF_PLACEMENT 2)
 
   // populations: two buffers, cur holds the current state
   void* f[2] = {nullptr, nullptr};
@@ -112,6 +113,15 @@ struct lbmpsm_ctx {
 };
 
 namespace {
+
+// Where a list's geometry and solid fractions run: 0 on the main stream
+// before its map; 1 on the second stream as soon as the list is given
+// (beside the previous step's collide and reduction); 2 on the second stream
+// once the previous step's collide has finished (beside its reduction).
+#ifndef LBMPSM_SF_PLACEMENT
+#define LBMPSM_SF_PLACEMENT 1
+#endif
+constexpr int kSfPlacement = LBMPSM_SF_PLACEMENT;
 
 int fail(lbmpsm_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -471,9 +481,17 @@ int run_map(lbmpsm_ctx* c, const DevList* dev = nullptr) {
   // this list's preparation reuses the brick lists, codes and work-list
   // counter of the previous map: it waits for it (the upload, if any, is
   // already queued behind the same wait)
+  hipStream_t sp = c->stream2;
+  if (kSfPlacement == 0) {  // on the main stream, after the upload
+    HIPCHK(c, hipEventRecord(c->ev_sf, c->stream2));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_sf, 0));
+    sp = c->stream;
+  } else if (kSfPlacement == 2) {  // after the last collide
+    HIPCHK(c, hipStreamWaitEvent(c->stream2, c->ev_collide, 0));
+  }
   {
-    ProfScope ps(c, "map_bin", c->stream2);  // counts are zero: set at create, reset by k_map
-    HIPCHK(c, launch_prep_bin(pa, c->stream2));
+    ProfScope ps(c, "map_bin", sp);  // counts are zero: set at create, reset by k_map
+    HIPCHK(c, launch_prep_bin(pa, sp));
   }
   MapArgs ma{};
   ma.g = c->g;
@@ -499,11 +517,13 @@ int run_map(lbmpsm_ctx* c, const DevList* dev = nullptr) {
   ma.slow_list = B.slow_list;
   ma.slow_count = B.slow_count;
   {
-    ProfScope ps(c, "map_fraction", c->stream2);
-    HIPCHK(c, launch_entry_sf(ma, c->stream2));
+    ProfScope ps(c, "map_fraction", sp);
+    HIPCHK(c, launch_entry_sf(ma, sp));
   }
-  HIPCHK(c, hipEventRecord(c->ev_sf, c->stream2));
-  HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_sf, 0));
+  if (kSfPlacement != 0) {
+    HIPCHK(c, hipEventRecord(c->ev_sf, c->stream2));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_sf, 0));
+  }
   {
     ProfScope ps(c, "map_particles");
     HIPCHK(c, launch_map(ma, B.nbricks, c->stream));
@@ -571,6 +591,7 @@ int step_launch(lbmpsm_ctx* c, int store, double* part, double* out, bool own_ou
     ProfScope ps(c, "collide_stream");
     HIPCHK(c, launch_collide(c->cfg.storage, a, c->stream));
   }
+  if (kSfPlacement == 2) HIPCHK(c, hipEventRecord(c->ev_collide, c->stream));
   c->cur = 1 - c->cur;
   c->poststream = 0;
   if (store) c->have_macros = 1;
@@ -742,7 +763,8 @@ int lbmpsm_create(const lbmpsm_config* cfg, lbmpsm_ctx** out) {
   }
   if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_sf, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_mapdone, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_mapdone, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_collide, hipEventDisableTiming) != hipSuccess) {
     lbmpsm_destroy(c);
     return fail(nullptr, LBMPSM_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
   }
@@ -843,6 +865,7 @@ int lbmpsm_destroy(lbmpsm_ctx* c) {
   if (c->stage_done) (void)hipEventDestroy(c->stage_done);
   if (c->ev_sf) (void)hipEventDestroy(c->ev_sf);
   if (c->ev_mapdone) (void)hipEventDestroy(c->ev_mapdone);
+  if (c->ev_collide) (void)hipEventDestroy(c->ev_collide);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

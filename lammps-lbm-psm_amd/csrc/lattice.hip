@@ -39,6 +39,9 @@
 #else
 #define LBMPSM_OCC
 #endif
+#ifndef LBMPSM_F64_SPLIT
+#define LBMPSM_F64_SPLIT 1  // 1: FP64 particle waves without a second particle skip its terms
+#endif
 #ifndef LBMPSM_LOWP_WAVES
 #define LBMPSM_LOWP_WAVES 5  // > 0: waves per SIMD asked of the bulk k_collide_lowp
 #endif
@@ -432,9 +435,11 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
     const uint8_t m = S.m;
     const bool psm = S.psm, o1 = S.o1, o2 = S.o2;
     double B1 = S.B1, B2 = S.B2;
-    const double* us1 = S.us1;
-    const double* us2 = S.us2;
-    double* Fa = S.Fa;
+    // scalar copies (kept in registers across the two forms of the particle
+    // pass below)
+    const double us1[3] = {S.us1[0], S.us1[1], S.us1[2]};
+    const double us2[3] = {S.us2[0], S.us2[1], S.us2[2]};
+    double Fa[6] = {S.Fa[0], S.Fa[1], S.Fa[2], S.Fa[3], S.Fa[4], S.Fa[5]};
     double rho, jx, jy, jz;
     moments<Q, double>(f, rho, jx, jy, jz);
     jx = jx + F0 * 0.5;
@@ -479,52 +484,72 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
         Btot = 1.0;
       }
       const double oneMinusOmega = 1.0 - omega;
-      Unroll<0, Q>::run([&]<int q>() {
-        constexpr double w = VelSet<Q>::w[q];
-        constexpr int cx = VelSet<Q>::ex[q], cy = VelSet<Q>::ey[q], cz = VelSet<Q>::ez[q];
-        const double fi = f[q];
-        const double eu = edot<Q, q>(ux, uy, uz);
-        const double feq = rho * w * (1.0 + eu * inv2 + eu * eu * 0.5 * inv4 - usqt);
-        double sc1 = 0.0, sc2 = 0.0;
-        if (o1) {
-          const double e1 = edot<Q, q>(us1[0], us1[1], us1[2]);
-          const double fs = rho * w * (1.0 + e1 * inv2 + 0.5 * e1 * e1 * inv4 - u1sq);
-          sc1 = fs - fi + oneMinusOmega * (fi - feq);
-        }
-        if (o2) {
-          const double e2 = edot<Q, q>(us2[0], us2[1], us2[2]);
-          const double fs = rho * w * (1.0 + e2 * inv2 + 0.5 * e2 * e2 * inv4 - u2sq);
-          sc2 = fs - fi + oneMinusOmega * (fi - feq);
-        }
-        double out = fi + (1.0 - Btot) * (feq - fi) * omega + B1 * sc1 + B2 * sc2;
-        if (a.guo) {
-          constexpr double ex = cx, ey = cy, ez = cz;
-          const double eF = edot<Q, q>(F0, F1, F2);
-          const double Fi =
-              w * (((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) * inv2 + (eu * inv4 * eF));
-          out = out + a.guo_pref * Fi;
-        }
-        st<ST_F64>(pop_ptr_w<ST_F64, Q>(a.fout, q, plane, rb, uc), out, 1.0f);
-        // Node force on each slot, src/LBM_PSM_BGK_dynamics.h:101-113.
-        if (o1) {
-          const double t = -B1 * sc1;
-          if constexpr (cx > 0) Fa[0] = Fa[0] + t;
-          if constexpr (cx < 0) Fa[0] = Fa[0] - t;
-          if constexpr (cy > 0) Fa[1] = Fa[1] + t;
-          if constexpr (cy < 0) Fa[1] = Fa[1] - t;
-          if constexpr (cz > 0) Fa[2] = Fa[2] + t;
-          if constexpr (cz < 0) Fa[2] = Fa[2] - t;
-        }
-        if (o2) {
-          const double t = -B2 * sc2;
-          if constexpr (cx > 0) Fa[3] = Fa[3] + t;
-          if constexpr (cx < 0) Fa[3] = Fa[3] - t;
-          if constexpr (cy > 0) Fa[4] = Fa[4] + t;
-          if constexpr (cy < 0) Fa[4] = Fa[4] - t;
-          if constexpr (cz > 0) Fa[5] = Fa[5] + t;
-          if constexpr (cz < 0) Fa[5] = Fa[5] - t;
-        }
-      });
+      // TWO: the second slot's terms are formed.  Without LBMPSM_F64_SPLIT
+      // every particle wave forms them (zero on lanes without a second
+      // particle); with it, a wave in which no lane holds a second particle
+      // adds the reference's B2 * solid_coll2 = +0.0 as a constant instead
+      // (x + 0.0 keeps every bit of x but the sign of a zero, as the
+      // reference's sum does).
+      auto pass = [&]<bool TWO>() __attribute__((always_inline)) {
+        Unroll<0, Q>::run([&]<int q>() {
+          constexpr double w = VelSet<Q>::w[q];
+          constexpr int cx = VelSet<Q>::ex[q], cy = VelSet<Q>::ey[q], cz = VelSet<Q>::ez[q];
+          const double fi = f[q];
+          const double eu = edot<Q, q>(ux, uy, uz);
+          const double feq = rho * w * (1.0 + eu * inv2 + eu * eu * 0.5 * inv4 - usqt);
+          double sc1 = 0.0, sc2 = 0.0;
+          if (o1) {
+            const double e1 = edot<Q, q>(us1[0], us1[1], us1[2]);
+            const double fs = rho * w * (1.0 + e1 * inv2 + 0.5 * e1 * e1 * inv4 - u1sq);
+            sc1 = fs - fi + oneMinusOmega * (fi - feq);
+          }
+          if constexpr (TWO) {
+            if (o2) {
+              const double e2 = edot<Q, q>(us2[0], us2[1], us2[2]);
+              const double fs = rho * w * (1.0 + e2 * inv2 + 0.5 * e2 * e2 * inv4 - u2sq);
+              sc2 = fs - fi + oneMinusOmega * (fi - feq);
+            }
+          }
+          double out = fi + (1.0 - Btot) * (feq - fi) * omega + B1 * sc1;
+          if constexpr (TWO) out = out + B2 * sc2;
+          else out = out + 0.0;
+          if (a.guo) {
+            constexpr double ex = cx, ey = cy, ez = cz;
+            const double eF = edot<Q, q>(F0, F1, F2);
+            const double Fi =
+                w * (((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) * inv2 + (eu * inv4 * eF));
+            out = out + a.guo_pref * Fi;
+          }
+          st<ST_F64>(pop_ptr_w<ST_F64, Q>(a.fout, q, plane, rb, uc), out, 1.0f);
+          // Node force on each slot, src/LBM_PSM_BGK_dynamics.h:101-113.
+          if (o1) {
+            const double t = -B1 * sc1;
+            if constexpr (cx > 0) Fa[0] = Fa[0] + t;
+            if constexpr (cx < 0) Fa[0] = Fa[0] - t;
+            if constexpr (cy > 0) Fa[1] = Fa[1] + t;
+            if constexpr (cy < 0) Fa[1] = Fa[1] - t;
+            if constexpr (cz > 0) Fa[2] = Fa[2] + t;
+            if constexpr (cz < 0) Fa[2] = Fa[2] - t;
+          }
+          if constexpr (TWO) {
+            if (o2) {
+              const double t = -B2 * sc2;
+              if constexpr (cx > 0) Fa[3] = Fa[3] + t;
+              if constexpr (cx < 0) Fa[3] = Fa[3] - t;
+              if constexpr (cy > 0) Fa[4] = Fa[4] + t;
+              if constexpr (cy < 0) Fa[4] = Fa[4] - t;
+              if constexpr (cz > 0) Fa[5] = Fa[5] + t;
+              if constexpr (cz < 0) Fa[5] = Fa[5] - t;
+            }
+          }
+        });
+      };
+#if LBMPSM_F64_SPLIT
+      if (__any(o2)) pass.template operator()<true>();
+      else pass.template operator()<false>();
+#else
+      pass.template operator()<true>();
+#endif
       if (o1) {
         a.Fh[n] = Fa[0];
         a.Fh[plane + n] = Fa[1];
