@@ -116,6 +116,17 @@ int lbmpsm_init_equilibrium(lbmpsm_ctx* ctx, double rho, double ux, double uy, d
  * Replaces fix_LBM_PSM_BC::post_force, src/fix_LBM_PSM_BC.cpp:151-402. */
 int lbmpsm_set_bc(lbmpsm_ctx* ctx, int keyword, double u_infty, double rho_outlet);
 
+/* lbmpsm_set_bc with the face ranges a run of the reference on a
+ * procgrid[0] x procgrid[1] x procgrid[2] rank grid applies: every rank
+ * applies the faces it holds over the same range of its own local indices
+ * (src/fix_LBM_PSM_BC.cpp:223-400), which for the 2D xFlow y faces
+ * ([1+e, nx-2-e], :239-242) leaves out the first and last interior node of
+ * every rank, not only of the lattice.  No counterpart in the reference (it
+ * is the reference's behaviour on that grid): for reproducing results the
+ * reference published from multi-rank runs.  {1,1,1} is lbmpsm_set_bc. */
+int lbmpsm_set_bc_ranks(lbmpsm_ctx* ctx, int keyword, double u_infty, double rho_outlet,
+                        const int procgrid[3]);
+
 /* Map n spheres (LAMMPS order, locals then ghosts) onto the lattice: solid
  * fraction, rigid-body node velocity and the two-slot assignment.
  * Replaces ExchangeParticleData::setParticlesOnLattice,

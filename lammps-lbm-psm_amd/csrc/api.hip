@@ -76,6 +76,7 @@ struct lbmpsm_ctx {
 
   // boundary faces
   int bc_keyword = 0;
+  int procgrid[3] = {1, 1, 1};  // reference rank grid whose face ranges are emulated
   int nfaces = 0;
   Face faces[kMaxFaces];
 
@@ -371,8 +372,48 @@ int size_codes(lbmpsm_ctx* c, int n, double rmax) {
 
 // Face table of `fix lbm-psm-bc`, src/fix_LBM_PSM_BC.cpp:223-401, in the
 // reference's halo'd coordinates, converted to stored coordinates.
+void add_face_one(lbmpsm_ctx* c, int kind, int axis, int plane_ref, int a_lo, int a_hi, int b_lo,
+                  int b_hi, double ux, double uy, double uz, double rho);
+
+// One face given by its range on one rank.  With a reference rank grid
+// (c->procgrid), every rank that holds the face applies it over the same
+// range of its own local indices (src/fix_LBM_PSM_BC.cpp:223-400 index the
+// rank's local lattice), so a range that leaves out interior nodes next to
+// the ends leaves them out at every rank boundary too: the face becomes one
+// segment per rank along each tangential axis.  Ranks split the interior
+// nodes of an axis as src/LBM_PSM_lattice.cpp:34-63 does (the last rank takes
+// the remainder).
 void add_face(lbmpsm_ctx* c, int kind, int axis, int plane_ref, int a_lo, int a_hi, int b_lo,
               int b_hi, double ux, double uy, double uz, double rho) {
+  const Geom& g = c->g;
+  const int ta = axis == 0 ? 1 : 0;
+  const int tb = axis == 2 ? 1 : 2;
+  const int nglob[3] = {g.nx, g.ny, g.dim == 3 ? g.nz : 1};
+  // shifts of the range from the interior [1, n] on one rank
+  const int sa_lo = a_lo - 1, sa_hi = nglob[ta] - a_hi;
+  const int sb_lo = b_lo - 1, sb_hi = nglob[tb] - b_hi;
+  const bool b_active = !(g.dim == 2 && tb == 2);
+  const int pa = c->procgrid[ta], pb = b_active ? c->procgrid[tb] : 1;
+  for (int ra = 0; ra < pa; ++ra) {
+    const int la = nglob[ta] / pa + (ra == pa - 1 ? nglob[ta] % pa : 0);
+    const int sa = 1 + ra * (nglob[ta] / pa);
+    for (int rb = 0; rb < pb; ++rb) {
+      int lo_b = b_lo, hi_b = b_hi;
+      if (b_active) {
+        const int lb = nglob[tb] / pb + (rb == pb - 1 ? nglob[tb] % pb : 0);
+        const int sb = 1 + rb * (nglob[tb] / pb);
+        lo_b = sb + sb_lo;
+        hi_b = sb + lb - 1 - sb_hi;
+      }
+      add_face_one(c, kind, axis, plane_ref, sa + sa_lo, sa + la - 1 - sa_hi, lo_b, hi_b, ux, uy,
+                   uz, rho);
+    }
+  }
+}
+
+void add_face_one(lbmpsm_ctx* c, int kind, int axis, int plane_ref, int a_lo, int a_hi, int b_lo,
+                  int b_hi, double ux, double uy, double uz, double rho) {
+  if (c->nfaces >= kMaxFaces) return;  // rejected by the caller (face_count)
   const Geom& g = c->g;
   const int ta = axis == 0 ? 1 : 0;         // first tangential axis
   const int tb = axis == 2 ? 1 : 2;         // second tangential axis
@@ -910,11 +951,29 @@ int lbmpsm_init_equilibrium(lbmpsm_ctx* c, double rho, double ux, double uy, dou
 }
 
 int lbmpsm_set_bc(lbmpsm_ctx* c, int keyword, double u_infty, double rho_outlet) {
-  if (!c) return LBMPSM_ERR_ARG;
+  const int one[3] = {1, 1, 1};
+  return lbmpsm_set_bc_ranks(c, keyword, u_infty, rho_outlet, one);
+}
+
+int lbmpsm_set_bc_ranks(lbmpsm_ctx* c, int keyword, double u_infty, double rho_outlet,
+                        const int procgrid[3]) {
+  if (!c || !procgrid) return LBMPSM_ERR_ARG;
   if (keyword < LBMPSM_BC_NONE || keyword > LBMPSM_BC_CLOSEDBOX)
     return fail(c, LBMPSM_ERR_ARG, "Illegal fix lbm-psm-bc command");
+  const int nglob[3] = {c->g.nx, c->g.ny, c->g.nz};
+  int64_t segs = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (procgrid[d] < 1 || procgrid[d] > nglob[d])
+      return fail(c, LBMPSM_ERR_ARG, "rank grid must have 1..n ranks per axis");
+    segs = std::max<int64_t>(segs, (int64_t)procgrid[d] * procgrid[(d + 1) % 3]);
+  }
+  if (c->g.dim == 2 && procgrid[2] != 1)
+    return fail(c, LBMPSM_ERR_ARG, "a 2D lattice has one rank along z");
+  if (6 * segs > kMaxFaces)
+    return fail(c, LBMPSM_ERR_LIMIT, "rank grid gives more face segments than the table holds");
   HIPCHK(c, hipSetDevice(c->device));
   c->bc_keyword = keyword;
+  for (int d = 0; d < 3; ++d) c->procgrid[d] = procgrid[d];
   build_faces(c, keyword, u_infty, rho_outlet);
   if (c->nfaces)
     HIPCHK(c, hipMemcpyAsync(c->d_faces, c->faces, c->nfaces * sizeof(Face), hipMemcpyHostToDevice, c->stream));

@@ -9,7 +9,9 @@
 
 namespace lbmpsm {
 
-constexpr int kMaxFaces = 6;
+// Faces of one `lbm-psm-bc` keyword (at most 6), or of its emulation on a
+// reference rank grid (one segment per rank along each tangential axis).
+constexpr int kMaxFaces = 256;
 
 // Stored lattice geometry.  Node (i,j,k) of the stored box sits at
 // n = (i*sy + j)*sz + k; each population is a plane of `plane` elements.

@@ -103,6 +103,7 @@ def _load() -> C.CDLL:
         "lbmpsm_get_info": (C.c_int, [P, C.POINTER(Info)]),
         "lbmpsm_init_equilibrium": (C.c_int, [P, C.c_double, C.c_double, C.c_double, C.c_double]),
         "lbmpsm_set_bc": (C.c_int, [P, C.c_int, C.c_double, C.c_double]),
+        "lbmpsm_set_bc_ranks": (C.c_int, [P, C.c_int, C.c_double, C.c_double, ip]),
         "lbmpsm_set_particles": (C.c_int, [P, C.c_int, ip, dp, dp, dp, dp]),
         "lbmpsm_set_particles_device": (C.c_int, [P, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                                   C.c_void_p, C.c_void_p, C.c_double]),

@@ -99,13 +99,18 @@ class Lattice:
     def init_equilibrium(self, rho=1.0, u=(0.0, 0.0, 0.0)):
         self._check(N.LIB.lbmpsm_init_equilibrium(self._h, rho, u[0], u[1], u[2]))
 
-    def set_bc(self, keyword, u_infty=None, rho_outlet=1.0):
+    def set_bc(self, keyword, u_infty=None, rho_outlet=1.0, procgrid=None):
         """`fix lbm-psm-bc keyword`; the speed defaults to the fix's u_lb
-        (src/fix_LBM_PSM_BC.cpp:156)."""
+        (src/fix_LBM_PSM_BC.cpp:156).  procgrid: the face ranges of a
+        reference run on that rank grid (lbmpsm_set_bc_ranks)."""
         if u_infty is None:
             u_infty = self.cfg.u_lb
         kw = N.BC_KEYWORDS[keyword] if isinstance(keyword, str) else int(keyword)
-        self._check(N.LIB.lbmpsm_set_bc(self._h, kw, u_infty, rho_outlet))
+        if procgrid is None:
+            self._check(N.LIB.lbmpsm_set_bc(self._h, kw, u_infty, rho_outlet))
+        else:
+            pg = np.ascontiguousarray(procgrid, dtype=np.int32)
+            self._check(N.LIB.lbmpsm_set_bc_ranks(self._h, kw, u_infty, rho_outlet, N.iptr(pg)))
 
     # -- one LBM step ---------------------------------------------------------
     def set_particles(self, tag, x, v, omega, radius):

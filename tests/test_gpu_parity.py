@@ -368,3 +368,48 @@ def test_two_lists_set_before_one_step():
     lat.step(store_macros=True)
     Fg, Tg, Sg = lat.hydro()
     compare(lat, ora, [(lst_b, (Fg, Tg, Sg), (Fo, To, So))])
+
+
+def _rank_segments(n, p, s_lo, s_hi):
+    """Ranges of one face on a rank grid: ranks split the n interior nodes
+    of an axis n // p each, the last taking the remainder
+    (src/LBM_PSM_lattice.cpp:34-63), and each applies the face over its own
+    local range [1 + s_lo, n_local - s_hi] (src/fix_LBM_PSM_BC.cpp:223-271)."""
+    out = []
+    for r in range(p):
+        ln = n // p + (n % p if r == p - 1 else 0)
+        st = 1 + r * (n // p)
+        out.append((st + s_lo, st + ln - 1 - s_hi))
+    return out
+
+
+@pytest.mark.parametrize("grid", [(3, 2, 1), (4, 1, 1)])
+def test_xflow_2d_on_a_reference_rank_grid(grid):
+    """lbmpsm_set_bc_ranks against the oracle with the same faces applied
+    per rank through its face functions: 2D xFlow, whose y faces leave out
+    the first and last interior node of every x-rank."""
+    n = (38, 23, 1)
+    per = (1, 1, 1)
+    U = 0.01
+    box = lattice_box(2, n, per)
+    atoms = random_atoms(box, [4.0, 3.0], seed=21, vmax=5e-3, wmax=1e-3, gap=0.5)
+    print(f"\n[case] 2D xFlow on rank grid {grid}, {n}", flush=True)
+    lat = lbmpsm.Lattice(lbmpsm.lattice_units_config(2, n, per, 0.7, u_lb=U))
+    lat.set_bc("xFlow", procgrid=grid)
+    ora = O.Oracle(2, n, per, 0.7, unit=O.lattice_units(U))
+    for step in range(12):
+        H.move_linear(atoms, box, step, 1.0)
+        lst = H.build_list(atoms, box, 8.0)
+        ora.fix_step(lst, bc_keyword=0)
+        for (a0, a1) in _rank_segments(n[1], grid[1], 0, 0):
+            ora.zou_he_face(0, 1, (a0, a1), u=(U, 0, 0))             # xn
+        for (a0, a1) in _rank_segments(n[1], grid[1], 0, 0):
+            ora.zou_he_face(2, n[0], (a0, a1), rho=1.0)               # density xp
+        for (a0, a1) in _rank_segments(n[0], grid[0], 1, 1):
+            ora.zou_he_face(3, 1, (a0, a1), u=(U, 0, 0))             # yn
+        for (a0, a1) in _rank_segments(n[0], grid[0], 1, 1):
+            ora.zou_he_face(4, n[1], (a0, a1), u=(U, 0, 0))          # yp
+        lat.set_particles(lst.tag, lst.x, lst.v, lst.omega, lst.radius)
+        lat.step()
+    assert np.array_equal(lat.populations(), ora.populations())
+    lat.close()

@@ -114,3 +114,120 @@ def test_shear_torque_and_stresslet_deck():
     assert abs(e_s - g["stresslet_error_percent"]) < 5e-3
     assert abs(e_t - g["torque_error_percent"]) < 5e-3
     lat.close()
+
+
+def _drag_2d_cd(procgrid):
+    lc, nlc, rho, nu, re, tau = 0.0005, 11, 1000.0, 1e-4, 0.1, 0.7
+    boxlo = (0.0, -0.0125, -0.00025)
+    prd = (0.07, 0.025, 0.0005)
+    cfg = lbmpsm.config_from_fix(2, nlc, lc, rho, nu, re, tau, (0, 0, 0), boxlo, prd, (1, 1, 1))
+    assert (cfg.nx, cfg.ny) == (1400, 500)
+    lat = lbmpsm.Lattice(cfg)
+    lat.set_bc("xFlow", procgrid=procgrid)
+    box = H.Box(lo=np.array(boxlo), prd=np.array(prd), periodic=(1, 1, 1), dim=2)
+    atoms = H.Atoms(tag=np.array([1], np.int32), x=np.array([[0.02, 0.0, 0.0]]),
+                    v=np.zeros((1, 3)), omega=np.zeros((1, 3)), radius=np.array([lc / 2.0]))
+    lst = H.build_list(atoms, box, cutoff=0.001)  # comm_modify cutoff 0.001
+    assert lst.n == 1
+    steps = 1_000_000
+    dumps = [0.0]  # step 0: no LBM step has run, fx = 0
+    print(f"\n[case] 2D drag deck Re 0.1, 1400x500, rank grid {procgrid}, {steps} steps",
+          flush=True)
+    for step in range(1, steps + 1):
+        lat.set_particles(lst.tag, lst.x, lst.v, lst.omega, lst.radius)
+        lat.step()
+        if step % 100 == 0:
+            if step >= 790_000:
+                F, _, _ = lat.hydro()
+                dumps.append(F[0, 0])
+            else:
+                dumps.append(None)
+        if step % 200_000 == 0:
+            print(f"  step {step}", flush=True)
+    lat.close()
+    f = np.array(dumps[int(len(dumps) * 0.8):], dtype=float)
+    Uc = re * nu / lc
+    dx = lc / (nlc - 1)
+    return abs(np.mean(f)) / (0.5 * rho * Uc ** 2 * lc * dx)
+
+
+def test_2d_drag_coefficient_deck_re0p1():
+    """verification/2D_dragForce/Re0p1/in.lbmpsm:12-41: a fixed disk of
+    diameter lc = 0.0005 at (0.02, 0) in a 0.07 x 0.025 periodic box
+    (1400 x 500 D2Q9 nodes), `lbm-psm-bc xFlow` (Zou-He velocity inlet and
+    density outlet in x, velocity faces in y), Re 0.1, tau 0.7, 1 000 000
+    steps with the force dumped every 100 steps; Cd = |mean fx over the last
+    20 % of the dumps| / (0.5 rho Uc^2 lc dx)
+    (verification/2D_dragForce/Re0p1/analyseDragForce.py:5-33) against
+    verification/2D_dragForce/dragCoefficients.dat:2 (85.2955).
+    Measured: 85.2881, 8.7e-5 below.  Not the cause: the deck's rank grid
+    (`processors 9 3 1`, whose per-rank xFlow y-face ranges leave out two
+    nodes per face at every inner x-rank boundary; emulated with
+    lbmpsm_set_bc_ranks it gives 85.2880) and the 2D dz = -1 term of the
+    solid fraction (the disk at z_loc = 0 gives the same 85.2881).  The
+    drag still drifts by 5e-4 across the averaging window, so the result is
+    sensitive to the transient.  A slip in the 2D faces moves Cd by far
+    more than the bound of 2e-4 (see the Zou-He pins of the oracle)."""
+    cd = _drag_2d_cd((1, 1, 1))
+    ref = GOLD["drag_coefficient_2d"]["Cd"][0]
+    print(f"  Cd {cd:.10f} (reference {ref:.10f}), rel diff {abs(cd - ref) / ref:.2e}", flush=True)
+    assert abs(cd - ref) / ref < 2e-4
+
+
+def test_free_rotation_stresslet_and_spin_deck():
+    """verification/3D_stresslet-tau0p7/in.lbmpsm:24-41: the sphere of the
+    torque deck, free to move under `fix nve/sphere` in `lbm-psm-bc shear`;
+    stresslet error (pressure virial p_xy h^3 against 20/3 pi mu a^3 E) and
+    spin error (omega_z against -Uc/h) at step 100000
+    (verification/3D_stresslet-tau0p7/analyseStressletAndAngularVelocity.py:
+    14-22, 56) against ErrorSummary.dat:2.
+    The integrator is LAMMPS's velocity-Verlet for spheres (FixNVESphere):
+    initial_integrate v += dtf f/m, x += dt v, omega += dtf torque/(0.4 m r^2)
+    with the previous step's f and torque; post_force (the LBM step) with the
+    half-step v and omega; final_integrate with the new f and torque;
+    dtf = dt/2 (SI units), m = 4/3 pi r^3 rho with `set density 1000`."""
+    lat, cfg = _lattice("shear")
+    dt = cfg.dt_d * cfg.tc
+    dtf = 0.5 * dt
+    r = LC / 2.0
+    m = 4.0 / 3.0 * math.pi * r ** 3 * 1000.0
+    inertia = 0.4 * m * r * r
+    box = H.Box(lo=np.zeros(3), prd=np.array([BOX, BOX, BOX]), periodic=(1, 0, 1))
+    atoms = H.Atoms(tag=np.array([1], np.int32), x=np.array([[0.0025, 0.0025, 0.0025]]),
+                    v=np.zeros((1, 3)), omega=np.zeros((1, 3)), radius=np.array([r]))
+    f = np.zeros((1, 3))
+    t = np.zeros((1, 3))
+    print(f"\n[case] free-rotation stresslet deck (nve/sphere), {STEPS} steps", flush=True)
+    for step in range(1, STEPS + 1):
+        atoms.v += dtf / m * f
+        atoms.x = H.remap(atoms.x + dt * atoms.v, box)
+        atoms.omega += dtf / inertia * t
+        lst = H.build_list(atoms, box, cutoff=0.001)
+        lat.set_particles(lst.tag, lst.x, lst.v, lst.omega, lst.radius)
+        lat.step()
+        F, T, S = lat.hydro()
+        f, t = H.reduce_to_owners(lst, F, T)
+        atoms.v += dtf / m * f
+        atoms.omega += dtf / inertia * t
+        if step % 20000 == 0:
+            print(f"  step {step}: omega_z {atoms.omega[0, 2]:.8f}", flush=True)
+    Uc = RE * NU / LC
+    h = BOX
+    E = 0.5 * Uc * 2.0 / h
+    stresslet = 20.0 / 3.0 * math.pi * NU * RHO * r ** 3 * E
+    S_xy = S[:, 3].sum()
+    e_s = abs(abs(S_xy) - stresslet) / stresslet * 100.0
+    w_ref = Uc / h
+    e_w = abs(abs(atoms.omega[0, 2]) - w_ref) / w_ref * 100.0
+    # the analysis reads omega_z from the dump, which LAMMPS writes with %g
+    # (6 significant digits): -3.99819 gives the published 0.04525 %
+    w_dump = float(f"{atoms.omega[0, 2]:g}")
+    e_w_dump = abs(abs(w_dump) - w_ref) / w_ref * 100.0
+    g = GOLD["stresslet_free_rotation_tau0p7"]
+    print(f"  stresslet error {e_s:.8f} % (reference {g['stresslet_error_percent']:.8f} %), "
+          f"spin error {e_w:.6f} % (as dumped {e_w_dump:.8f} %, reference "
+          f"{g['omega_error_percent']:.8f} %)", flush=True)
+    assert atoms.omega[0, 2] < 0.0
+    assert abs(e_s - g["stresslet_error_percent"]) < 1e-5
+    assert abs(e_w_dump - g["omega_error_percent"]) < 1e-9
+    lat.close()
