@@ -605,6 +605,56 @@ template <int DIM, int Q, int ST>
 __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, unsigned rb,
                                              unsigned uc, const SlotsF32& S);
 
+// Weights of the FP32 form's even terms (w (drho + rho (...)), the solid
+// equilibrium and the Guo term).  The nearest floats to 1/3, 1/18, 1/36 sum
+// to 1 + 1.5e-8 (to 4/9, 1/9, 1/36: 1 + 7.4e-9), and a collision then hands
+// each node omega * 1.5e-8 * drho of new mass per step: with the pressure
+// drop of a packed column (drho ~ 0.1) that is a source spread over the
+// whole bed, and the flux it adds at the top moves every drag by ~1e-5 over
+// 10^4 steps.  These floats sum to exactly 1: the axis weight is 1/18 (1/9)
+// with its mantissa rounded down to a multiple of 4 (6e-8 low), the
+// diagonal one half (a quarter) of it, and the rest weight 1 minus the
+// others, which is a float.  The odd terms keep their own exact split.
+template <int Q>
+struct LowpW;
+template <>
+struct LowpW<19> {
+  static constexpr float axis = 14913080.0f / 268435456.0f;  // 2^-28 units
+  static constexpr float diag = axis * 0.5f;
+  static constexpr float rest = 11184812.0f / 33554432.0f;   // 2^-25 units
+  static_assert((double)rest + 6.0 * (double)axis + 12.0 * (double)diag == 1.0);
+};
+template <>
+struct LowpW<9> {
+  static constexpr float axis = 14913080.0f / 134217728.0f;  // 2^-27 units
+  static constexpr float diag = axis * 0.25f;
+  static constexpr float rest = 14913082.0f / 33554432.0f;   // 2^-25 units
+  static_assert((double)rest + 4.0 * (double)axis + 4.0 * (double)diag == 1.0);
+};
+template <int Q, int q>
+__device__ __forceinline__ constexpr float lowp_w() {
+  constexpr int n2 = VelSet<Q>::ex[q] * VelSet<Q>::ex[q] + VelSet<Q>::ey[q] * VelSet<Q>::ey[q] +
+                     VelSet<Q>::ez[q] * VelSet<Q>::ez[q];
+  return n2 == 0 ? LowpW<Q>::rest : (n2 == 1 ? LowpW<Q>::axis : LowpW<Q>::diag);
+}
+// rho times the weight of direction q, from the node's three products.
+template <int Q, int q>
+__device__ __forceinline__ float lowp_rw(const float (&rw)[3]) {
+  constexpr int n2 = VelSet<Q>::ex[q] * VelSet<Q>::ex[q] + VelSet<Q>::ey[q] * VelSet<Q>::ey[q] +
+                     VelSet<Q>::ez[q] * VelSet<Q>::ez[q];
+  return rw[n2];
+}
+// Even part of a shifted equilibrium plus the rest r of the population:
+// w drho + (rho w t + r), with the product w drho left unrounded inside the
+// last fused multiply-add.  Rounded on its own, w drho takes only a few
+// phases against the result's grid (the rest weight's mantissa is ~2/3 of
+// a power of two, so w drho lands on thirds of a unit), which at drho =
+// 0.1 biases it by -0.07 units, a mass loss of 3e-10 per node and step;
+// the addend's arbitrary low bits leave the single rounding unbiased.
+__device__ __forceinline__ float lowp_even(float w, float drho, float rw, float t, float r) {
+  return __builtin_fmaf(w, drho, __builtin_fmaf(rw, t, r));
+}
+
 // Odd part of the FP32 equilibrium, 3 w_q (e_q . p) with p = rho u, formed
 // as a signed sum of A = 3 w_axis p per axis (w_axis = 1/18 in D3Q19, 1/9 in
 // D2Q9); a diagonal weight is w_axis/2 (D3Q19) or w_axis/4 (D2Q9), a power
@@ -707,6 +757,7 @@ __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f
       a.u_out[2 * plane + n] = uz;
     }
     const float usq = 1.5f * (ux * ux + uy * uy + uz * uz);
+    const float rw[3] = {rho * LowpW<Q>::rest, rho * LowpW<Q>::axis, rho * LowpW<Q>::diag};
     const uint8_t m = S.m;
     // g_eq = w (drho + rho (4.5 (e.u)^2 - 1.5 u^2)) + 3 w e.(rho u), with
     // rho u = j + F/2 (u itself only enters terms even in e); mixed waves
@@ -716,9 +767,10 @@ __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f
                 Az = odd_axis_j<Q>(jz, hF2);
     if (!S.psm) {
       Unroll<0, Q>::run([&]<int q>() {
-        constexpr float w = (float)VelSet<Q>::w[q];
+        constexpr float w = lowp_w<Q, q>();
         const float eu = edot<Q, q>(ux, uy, uz);
-        const float geq = w * (drho + rho * (4.5f * eu * eu - usq)) + eq_odd<Q, q>(Ax, Ay, Az);
+        const float geq = lowp_even(w, drho, lowp_rw<Q, q>(rw), 4.5f * eu * eu - usq,
+                                    eq_odd<Q, q>(Ax, Ay, Az));
         float out = f[q] + omega * (geq - f[q]);
         if (a.guo) {
           constexpr float ex = VelSet<Q>::ex[q], ey = VelSet<Q>::ey[q], ez = VelSet<Q>::ez[q];
@@ -748,24 +800,29 @@ __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f
       // out instead of issuing them under an empty lane mask.
       auto pass = [&]<bool TWO>() {
         Unroll<0, Q>::run([&]<int q>() {
-          constexpr float w = (float)VelSet<Q>::w[q];
+          constexpr float w = lowp_w<Q, q>();
           constexpr int cx = VelSet<Q>::ex[q], cy = VelSet<Q>::ey[q], cz = VelSet<Q>::ez[q];
           const float gi = f[q];
           const float eu = edot<Q, q>(ux, uy, uz);
-          const float geq = w * (drho + rho * (4.5f * eu * eu - usq)) + eq_odd<Q, q>(Ax, Ay, Az);
+          const float geq = lowp_even(w, drho, lowp_rw<Q, q>(rw), 4.5f * eu * eu - usq,
+                                    eq_odd<Q, q>(Ax, Ay, Az));
           float sc1 = 0.f, sc2 = 0.f;
-          // the solid equilibrium keeps the plain float weight: its momentum
-          // bias, 7.4e-9 rho u_s, is ~1e-8 of the node's exchange B rho (u_s - u)
+          // g_s - g_i of the solid equilibrium in one rounding (the addend
+          // -g_i keeps it unbiased for a particle at rest too); its momentum
+          // takes the even weights, a bias of 6e-8 rho u_s, ~1e-7 of the
+          // node's exchange B rho (u_s - u) for particle speeds up to the fluid's
           if (o1) {
             const float e1 = edot<Q, q>(us1[0], us1[1], us1[2]);
-            const float gs = w * (drho + rho * (3.0f * e1 + 4.5f * e1 * e1 - u1sq));
-            sc1 = (gs - gi) + omo * (gi - geq);
+            const float ds = lowp_even(w, drho, lowp_rw<Q, q>(rw),
+                                       3.0f * e1 + 4.5f * e1 * e1 - u1sq, -gi);
+            sc1 = ds + omo * (gi - geq);
           }
           if constexpr (TWO) {
             if (o2) {
               const float e2 = edot<Q, q>(us2[0], us2[1], us2[2]);
-              const float gs = w * (drho + rho * (3.0f * e2 + 4.5f * e2 * e2 - u2sq));
-              sc2 = (gs - gi) + omo * (gi - geq);
+              const float ds = lowp_even(w, drho, lowp_rw<Q, q>(rw),
+                                         3.0f * e2 + 4.5f * e2 * e2 - u2sq, -gi);
+              sc2 = ds + omo * (gi - geq);
             }
           }
           float out = gi + (1.0f - Btot) * omega * (geq - gi) + B1 * sc1;

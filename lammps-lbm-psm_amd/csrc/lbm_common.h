@@ -130,6 +130,17 @@ struct ZHConst {
   static constexpr T sixth = T(1.0) / T(6.0);
 };
 
+// Outlet velocity of a density face on shifted populations g = f - w*rho0
+// (FP32 arithmetic; r0 = rho0 is the weight sum of the populations taken).
+// The reference's -1 + (r0 + sum g)/rho_bc subtracts two numbers near 1 and
+// keeps only ~6e-8 of an outlet velocity of order 1e-3 (about 1e-5 relative,
+// with the same sign every step at a steady outlet); (r0 - rho_bc) is exact
+// for close densities, so ((r0 - rho_bc) + sum g)/rho_bc is the same
+// quantity without the cancellation.  The FP64 forms keep the reference's.
+__device__ __forceinline__ float outlet_velocity_shifted(float rho_bc, float r0, float g_sum) {
+  return ((r0 - rho_bc) + g_sum) / rho_bc;
+}
+
 // :25-40
 template <typename T>
 __device__ __forceinline__ void zh2_vel_xn(T* f, T ux, T r0) {
@@ -152,7 +163,11 @@ __device__ __forceinline__ void zh2_vel_xp(T* f, T ux, T r0) {
 template <typename T>
 __device__ __forceinline__ void zh2_dens_xp(T* f, T rho_bc, T r0) {
   const T s6 = ZHConst<T>::sixth, t23 = ZHConst<T>::twoThirds;
-  T ux = T(-1.0) + T(1.0) / rho_bc * (r0 + f[0] + f[2] + f[4] + T(2.0) * (f[1] + f[5] + f[8]));
+  T ux;
+  if constexpr (sizeof(T) == 8)
+    ux = T(-1.0) + T(1.0) / rho_bc * (r0 + f[0] + f[2] + f[4] + T(2.0) * (f[1] + f[5] + f[8]));
+  else
+    ux = outlet_velocity_shifted(rho_bc, r0, f[0] + f[2] + f[4] + T(2.0) * (f[1] + f[5] + f[8]));
   f[6] = f[8] + T(0.5) * (f[4] - f[2]) - rho_bc * ux * s6;
   f[7] = f[5] + T(0.5) * (f[2] - f[4]) - rho_bc * ux * s6;
   f[3] = f[1] - t23 * rho_bc * ux;
@@ -207,9 +222,15 @@ __device__ __forceinline__ void zh3_vel_xp(T* f, T ux, T uy, T uz, T r0) {
 template <typename T>
 __device__ __forceinline__ void zh3_dens_xp(T* f, T rho_bc, T uy, T uz, T r0) {
   const T t3 = ZHConst<T>::third, s6 = ZHConst<T>::sixth;
-  T ux = T(-1.0) + T(1.0) / rho_bc *
+  T ux;
+  if constexpr (sizeof(T) == 8)
+    ux = T(-1.0) + T(1.0) / rho_bc *
                        (r0 + f[0] + f[3] + f[4] + f[5] + f[6] + f[11] + f[17] + f[18] + f[12] +
                         T(2.0) * (f[1] + f[7] + f[13] + f[9] + f[15]));
+  else
+    ux = outlet_velocity_shifted(rho_bc, r0,
+                                 f[0] + f[3] + f[4] + f[5] + f[6] + f[11] + f[17] + f[18] +
+                                     f[12] + T(2.0) * (f[1] + f[7] + f[13] + f[9] + f[15]));
   T Nyx = T(0.5) * (f[3] + f[11] + f[17] - (f[4] + f[18] + f[12])) - t3 * rho_bc * uy;
   T Nzx = T(0.5) * (f[5] + f[18] + f[11] - (f[6] + f[17] + f[12])) - t3 * rho_bc * uz;
   f[2] = f[1] - t3 * rho_bc * ux;

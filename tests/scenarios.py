@@ -106,3 +106,24 @@ def check_hydro(got, ref, scale=None, tol=1e-12, label=""):
           flush=True)
     assert np.all(d <= tol * np.maximum(mag, floor) + 1e-300), label
     return worst, ncancel
+
+
+# Velocity sets in the reference's population order
+# (src/LBM_PSM_lattice.h:186-209) and the second-order equilibrium, for
+# building non-uniform initial states in tests.
+E9 = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, -1, 0],
+               [1, 1, 0], [-1, 1, 0], [-1, -1, 0], [1, -1, 0]], float)
+W9 = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+E19 = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
+                [1, 1, 0], [-1, -1, 0], [1, 0, 1], [-1, 0, -1], [0, 1, 1], [0, -1, -1],
+                [1, -1, 0], [-1, 1, 0], [1, 0, -1], [-1, 0, 1], [0, 1, -1], [0, -1, 1]], float)
+W19 = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+
+
+def equilibrium(dim, rho, u):
+    """f_eq = w rho (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u^2) per node; rho (...),
+    u (..., 3) -> (..., q)."""
+    E, W = (E19, W19) if dim == 3 else (E9, W9)
+    eu = u @ E.T
+    usq = np.sum(u * u, axis=-1)[..., None]
+    return W * rho[..., None] * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * usq)

@@ -6,11 +6,11 @@
   the velocity field within 1e-12 relative.
 * D3Q19 fluidised-bed column 1024 x 256 x 256 (x vertical), xFlow: Zou-He
   velocity inlet U = 0.005 at x = 0 and density outlet, periodic y/z,
-  gravity Fx = -2e-6, 20000 fixed spheres d = 12, tau 0.7, FP64 and FP32
-  storage: FP32 drag/torque within 1e-5 of FP64 after 300 steps, and the
-  step rate of both forms.  The spheres are seeded over the whole column
-  (phi = 0.27): 20000 of them in its lower third would need phi = 0.81,
-  beyond random sequential addition.
+  gravity Fx = -2e-6, 20000 spheres d = 12 moving linearly, tau 0.7, FP64
+  and FP32 storage for 10 000 steps: FP32 drag/torque within 1e-5 of FP64
+  (measured 1.0e-6 / 1.4e-6), and the step rate of both forms.  The spheres are seeded over the whole
+  column (phi = 0.27): 20000 of them in its lower third would need
+  phi = 0.81, beyond random sequential addition.
 * D3Q19 256^3 periodic, Fx = 5e-6, 30 % volume fraction of fixed spheres with
   d ~ U[10, 20] (seeded RSA, largest first), tau in {0.6, 0.65, 0.8}, 3000
   steps in FP64, FP32 and FP16 storage: per-sphere drag, torque and
@@ -18,7 +18,7 @@
   value.  FP32 within 1e-5 (measured 0.6e-6 to 3.6e-6: the flow here is
   driven by a body force of 5e-6 per node and step, so any rounding bias of
   the momentum a collision hands on shows in the drag; the FP32 equilibrium
-  is formed to carry none), FP16 within 2e-2 (measured up to 2e-3).
+  is formed to carry none), FP16 within 5e-3 (measured up to 2.0e-3).
 """
 import numpy as np
 import pytest
@@ -95,7 +95,7 @@ def test_dense_polydisperse_reduced_storage(tau):
         lat.close()
         print(f"  {storage}: mean drag {np.mean(Fo[:, 0]):.6e}", flush=True)
     F64, T64, S64 = res["f64"]
-    for storage, bound in (("f32", 1e-5), ("f16", 2e-2)):
+    for storage, bound in (("f32", 1e-5), ("f16", 5e-3)):
         F, T, S = res[storage]
         eF = np.max(np.linalg.norm(F - F64, axis=1)) / np.max(np.linalg.norm(F64, axis=1))
         eT = np.max(np.linalg.norm(T - T64, axis=1)) / np.max(np.linalg.norm(T64, axis=1))
@@ -104,79 +104,48 @@ def test_dense_polydisperse_reduced_storage(tau):
         assert eF <= bound and eT <= bound and eS <= bound
 
 
-def _rsa_grid(L, r, n, seed, lo_x, hi_x):
-    """Random sequential addition of n equal spheres (radius r) in a box
-    periodic in y and z, centres in [lo_x, hi_x) along x, with a cell grid."""
-    rng = np.random.default_rng(seed)
-    cell = 2.0 * r
-    nc = [max(1, int(L[d] // cell)) for d in range(3)]
-    grid = {}
-    pos = []
-    tries = 0
-    while len(pos) < n:
-        tries += 1
-        assert tries < 50 * n, "RSA jammed"
-        c = np.array([rng.uniform(lo_x, hi_x), rng.uniform(0, L[1]), rng.uniform(0, L[2])])
-        ci = [int(c[d] // cell) % nc[d] for d in range(3)]
-        ok = True
-        for dx in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dz in (-1, 0, 1):
-                    key = ((ci[0] + dx) % nc[0], (ci[1] + dy) % nc[1], (ci[2] + dz) % nc[2])
-                    for q in grid.get(key, ()):
-                        d = c - pos[q]
-                        d[1] -= L[1] * round(d[1] / L[1])
-                        d[2] -= L[2] * round(d[2] / L[2])
-                        if d @ d < (2.0 * r) ** 2:
-                            ok = False
-                            break
-                    if not ok:
-                        break
-                if not ok:
-                    break
-            if not ok:
-                break
-        if ok:
-            grid.setdefault(tuple(ci), []).append(len(pos))
-            pos.append(c)
-    return np.array(pos)
-
-
 @pytest.mark.slow
-def test_fluidised_bed_xflow_fp32_vs_fp64():
+def test_fluidised_bed_10000_steps_fp32_vs_fp64():
+    """The fluidised-bed column as the configuration names it: 10 000 steps
+    in FP64 and FP32 storage, 20 000 spheres d = 12 moving linearly (new
+    positions mapped every step), xFlow inlet U = 0.005 / outlet rho = 1,
+    gravity Fx = -2e-6, tau 0.7.  The FP32 drag and torque of every sphere
+    within 1e-5 of FP64 (relative to the largest; measured 1.0e-6 and
+    1.4e-6), the step rate of both.  The pressure drop over the bed holds
+    the density 0.1 above the stored shift rho0 = 1 at the bottom, so the
+    FP32 collision must keep mass exactly at drho != 0: with the nearest
+    floats as lattice weights (summing to 1 + 1.5e-8) the column gains a
+    spread mass source and every drag moves by 1.2e-5 (mean drag +5.6e-5)."""
     import time
-    n = (1024, 256, 256)
-    per = (0, 1, 1)
-    box = lattice_box(3, n, per)
-    L = box.prd
-    pos = _rsa_grid(L, 6.0, 20000, 505, 20.0, L[0] - 20.0)
-    atoms = H.Atoms(tag=np.arange(1, len(pos) + 1, dtype=np.int32), x=pos,
-                    v=np.zeros((len(pos), 3)), omega=np.zeros((len(pos), 3)),
-                    radius=np.full(len(pos), 6.0))
-    lst = H.build_list(atoms, box, 10.0)
-    print(f"\n[case] D3Q19 1024x256x256 xFlow U=0.005, Fx=-2e-6, {atoms.n} spheres d=12 "
-          f"({lst.n} entries), tau 0.7, 300 steps x FP64/FP32", flush=True)
+    from lbmpsm import workloads as W
+    c = W.BED
+    box, atoms = W.fluidised_bed()
+    steps = 10_000
+    print(f"\n[case] D3Q19 1024x256x256 xFlow, {atoms.n} moving spheres d=12, tau 0.7, "
+          f"{steps} steps x FP64/FP32", flush=True)
     res = {}
     for storage in ("f64", "f32"):
-        lat = lbmpsm.Lattice(lbmpsm.lattice_units_config(3, n, per, 0.7, F_lbm=(-2e-6, 0.0, 0.0),
-                                                         storage=storage, u_lb=0.005))
+        lat = lbmpsm.Lattice(lbmpsm.lattice_units_config(3, c["n"], c["periodic"], c["tau"],
+                                                         F_lbm=c["F"], storage=storage,
+                                                         u_lb=c["u_inlet"]))
         lat.set_bc("xFlow")
-        lat.set_particles(lst.tag, lst.x, lst.v, lst.omega, lst.radius)
-        for step in range(250):
-            lat.step()
-        lat.sync()
+        a = atoms.copy()
         t0 = time.perf_counter()
-        for step in range(50):
+        for step in range(steps):
+            H.move_linear(a, box, step, 1.0)
+            lst = H.build_list(a, box, c["radius"] + 4.0)
             lat.set_particles(lst.tag, lst.x, lst.v, lst.omega, lst.radius)
             lat.step()
-        lat.sync()
-        dt = (time.perf_counter() - t0) / 50
+            if (step + 1) % 2500 == 0:
+                print(f"  {storage} step {step + 1}", flush=True)
         F, T, S = lat.hydro()
+        dt = (time.perf_counter() - t0) / steps
         Fo, To = H.reduce_to_owners(lst, F, T)
         res[storage] = (Fo, To)
         lat.close()
-        print(f"  {storage}: {n[0] * n[1] * n[2] / dt / 1e6:.0f} MLUPS ({dt * 1e3:.2f} ms/step, "
-              f"host list each step), mean drag {np.mean(Fo[:, 0]):.6e}", flush=True)
+        print(f"  {storage}: {c['n'][0] * c['n'][1] * c['n'][2] / dt / 1e6:.0f} MLUPS "
+              f"({dt * 1e3:.2f} ms/step, host list every step), mean drag "
+              f"{np.mean(Fo[:, 0]):.6e}", flush=True)
         assert np.all(np.isfinite(Fo))
     F64, T64 = res["f64"]
     F32, T32 = res["f32"]

@@ -8,14 +8,16 @@ error of each reduced-precision result is measured against it.
 
 Bars: FP32 drag and torque within 1e-5 (relative to the largest particle
 force / torque), velocity field within 1e-5 of the largest velocity, mass
-drift below 1e-6 per node over the run.  FP16 is reported with bars ten
-times looser than its storage rounding (2^-11 of the stored deviations).
+drift below 1e-6 per node over the run.  FP16: drag, torque and velocity
+within 2e-3 (measured values are printed; the storage rounds each stored
+deviation to 2^-11 of itself), mass drift below 1e-4 per node.
 """
 import numpy as np
 import pytest
 
 import lbmpsm
 from lbmpsm import harness as H
+import scenarios as S
 from scenarios import lattice_box, random_atoms
 
 pytestmark = pytest.mark.gpu
@@ -47,7 +49,7 @@ def ref64():
     return run("f64")
 
 
-@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f16", 2e-2)])
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f16", 2e-3)])
 def test_reduced_storage_error(ref64, storage, tol):
     print(f"\n[case] precision {storage}", flush=True)
     F, T, S, u, m = run(storage)
@@ -89,7 +91,7 @@ def run_contacts(storage, steps=120):
     return F, T, shared
 
 
-@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f16", 2e-2)])
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f16", 2e-3)])
 def test_reduced_storage_with_shared_nodes(storage, tol):
     print(f"\n[case] precision {storage} with shared nodes, 48^3, 3 sphere pairs 0.3 apart",
           flush=True)
@@ -100,3 +102,41 @@ def test_reduced_storage_with_shared_nodes(storage, tol):
     eT = np.max(np.linalg.norm(T - T0, axis=1)) / np.max(np.linalg.norm(T0, axis=1))
     print(f"  {storage}: {shared} shared nodes, drag {eF:.2e} torque {eT:.2e}", flush=True)
     assert eF <= tol and eT <= tol
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fp32_mass_kept_at_raised_density(dim):
+    """FP32 storage keeps the mass of a periodic box whose density sits 0.1
+    above the stored shift rho0 = 1 (as the bottom of a packed column under
+    its pressure drop does).  Equilibrium of rho = 1.1 plus long-wave density
+    and velocity modes (amplitudes 1e-2 and 2e-3, which outlive the 500 steps
+    run here, so neighbouring nodes never hold the same values), no force, no
+    particles.  Collisions conserve mass, so the total may change only by the
+    rounding of the stored values, which then has no preferred sign (~1e-10
+    per node).  Two roundings would give it one: lattice weights whose floats
+    do not sum to 1 (the nearest floats sum to 1 + 1.5e-8 in D3Q19,
+    1 + 7.4e-9 in D2Q9) add omega * eps * drho per node and step, 1.1e-6
+    (5.3e-7) per node over this run; and the product of the D3Q19 rest
+    weight with drho, rounded on its own, falls on thirds of the result's
+    unit and is biased by -0.07 of it at drho = 0.1, 1.7e-7 per node here.
+    The remaining systematic term, 6e-8 * 1.5 u^2 * omega per node and step
+    from the float axis weight, is below 1e-9 here."""
+    n = (48, 48, 48) if dim == 3 else (256, 256, 1)
+    print(f"\n[case] FP32 mass at rho 1.1, D{dim}Q{19 if dim == 3 else 9} {n}, 500 steps",
+          flush=True)
+    x, y, z = np.meshgrid(*[2 * np.pi * np.arange(m) / m for m in n], indexing="ij")
+    rho = 1.1 + 1e-2 * np.sin(x + 2 * y + 0.3) * np.cos(z - 0.7)
+    u = np.zeros(n + (3,))
+    u[..., 0] = 2e-3 * np.sin(y + 0.1) * np.cos(2 * z + 0.2)
+    u[..., 1] = 2e-3 * np.cos(x - 0.4) * np.sin(z + 0.5)
+    if dim == 3:
+        u[..., 2] = 2e-3 * np.sin(2 * x + 0.6) * np.cos(y - 0.3)
+    lat = lbmpsm.Lattice(lbmpsm.lattice_units_config(dim, n, (1, 1, 1), 0.7, storage="f32"))
+    lat.set_populations(S.equilibrium(dim, rho, u))
+    m0 = lat.mass()
+    for _ in range(500):
+        lat.step()
+    drift = abs(lat.mass() - m0) / (n[0] * n[1] * n[2])
+    lat.close()
+    print(f"  mass drift per node {drift:.3e}", flush=True)
+    assert drift < 2e-8
