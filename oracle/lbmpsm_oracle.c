@@ -46,6 +46,10 @@ struct lbo_lattice {
   double* sf;     /* [node*2+s] solidFraction     */
   double* us;     /* [node*6+3s+d] particleVelocity */
   double* Fh;     /* [node*6+3s+d] hydrodynamicForce */
+  /* per axis, the lattice offset the map and the reduction subtract from a
+   * particle's global lattice position: (nTotal - p*2*envelopeWidth - 1),
+   * src/LBM_PSM_exchangeParticleData.cpp:25-35, 176-183; -1 on one rank */
+  double roff[3];
 };
 
 /* src/LBM_PSM_unitConversion.cpp:15-38 */
@@ -97,6 +101,7 @@ lbo_lattice* lbo_create(int dim, int nx, int ny, int nz, const int periodic[3],
   L->tau = tau;
   L->omega = 1.0 / tau; /* src/LBM_PSM_BGK_dynamics.cpp:16 */
   for (int d = 0; d < 3; ++d) L->F[d] = F_lbm[d];
+  for (int d = 0; d < 3; ++d) L->roff[d] = (0 - ((double)0 * 2.0 * (double)1) - 1);
   L->Fmag2 = 0.0;
   size_t nodes = (size_t)L->nx * L->ny * L->nz;
   L->f_curr = (double*)calloc(nodes * L->q, sizeof(double));
@@ -299,14 +304,13 @@ static inline int maxi(int a, int b) { return a > b ? a : b; }
 void lbo_set_particles(lbo_lattice* L, const lbo_units* U, const double origin[3], int n,
                        const int* tag, const double* x, const double* v,
                        const double* omega, const double* radius) {
-  const double off = (0 - ((double)0 * 2.0 * (double)1) - 1);
   for (int p = 0; p < n; ++p) {
     double x_lb_global = (x[3 * p + 0] - origin[0]) / U->dx;
     double y_lb_global = (x[3 * p + 1] - origin[1]) / U->dx;
     double z_lb_global = (x[3 * p + 2] - origin[2]) / U->dx;
-    double x_loc = x_lb_global - off;
-    double y_loc = y_lb_global - off;
-    double z_loc = z_lb_global - off;
+    double x_loc = x_lb_global - L->roff[0];
+    double y_loc = y_lb_global - L->roff[1];
+    double z_loc = z_lb_global - L->roff[2];
     double r_lb = radius[p] / U->dx;
     int zone[3][2] = {{(int)(x_loc - r_lb) - 2, (int)(x_loc + r_lb) + 2},
                       {(int)(y_loc - r_lb) - 2, (int)(y_loc + r_lb) + 2},
@@ -770,10 +774,9 @@ void lbo_hydro(const lbo_lattice* L, const lbo_units* U, const double origin[3],
                const double xPart[3], double rp, double* fHydro, double* tHydro,
                double* stresslet) {
   const int e = 1;
-  const double off = (0 - ((double)0 * 2.0 * (double)1) - 1);
-  double x_loc = (xPart[0] - origin[0]) / U->dx - off;
-  double y_loc = (xPart[1] - origin[1]) / U->dx - off;
-  double z_loc = (xPart[2] - origin[2]) / U->dx - off;
+  double x_loc = (xPart[0] - origin[0]) / U->dx - L->roff[0];
+  double y_loc = (xPart[1] - origin[1]) / U->dx - L->roff[1];
+  double z_loc = (xPart[2] - origin[2]) / U->dx - L->roff[2];
   double r_lb = rp / U->dx;
   int zone[3][2] = {{(int)(x_loc - r_lb) - 2, (int)(x_loc + r_lb) + 2},
                     {(int)(y_loc - r_lb) - 2, (int)(y_loc + r_lb) + 2},
@@ -837,6 +840,17 @@ void lbo_fix_step(lbo_lattice* L, const lbo_units* U, const double origin[3], in
     if (S) for (int d = 0; d < 6; ++d) S[6 * p + d] = st[d];
   }
   lbo_apply_bc(L, bc_keyword, U->u_lb);
+}
+
+/* One MPI rank's sub-domain of a decomposed run: ntotal[a] is the sum of
+ * the halo'd local sizes of the ranks below this one along axis a and
+ * pcoord[a] its coordinate in the rank grid (src/LBM_PSM_lattice.cpp:69-105,
+ * src/LBM_PSM_exchangeParticleData.cpp:25-35).  The halos of such a lattice
+ * are filled by the caller, as src/LBM_PSM_MPICOMM.h:79-147 does between
+ * ranks; create it with no periodic axis so lbo_halo_exchange leaves them. */
+void lbo_set_rank(lbo_lattice* L, const int ntotal[3], const int pcoord[3]) {
+  for (int d = 0; d < 3; ++d)
+    L->roff[d] = (ntotal[d] - ((double)pcoord[d] * 2.0 * (double)1) - 1);
 }
 
 double lbo_mass(const lbo_lattice* L) {

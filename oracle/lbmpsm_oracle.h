@@ -96,6 +96,10 @@ double* lbo_slot_u(lbo_lattice* L);   /* [nodes*6]                          */
 double* lbo_slot_F(lbo_lattice* L);   /* [nodes*6]                          */
 
 /* Sum of all populations over interior nodes (mass). */
+/* Make L one rank's sub-domain of a decomposed run (the map and reduction
+ * offsets of that rank); its halos are then the caller's to fill. */
+void lbo_set_rank(lbo_lattice* L, const int ntotal[3], const int pcoord[3]);
+
 double lbo_mass(const lbo_lattice* L);
 
 /* Number of OpenMP threads the collide loop uses (1 when built without). */

@@ -65,6 +65,7 @@ def _load():
         "lbo_slot_u": (dp, [P]),
         "lbo_slot_F": (dp, [P]),
         "lbo_mass": (C.c_double, [P]),
+        "lbo_set_rank": (None, [P, ip, ip]),
         "lbo_threads": (C.c_int, []),
         "lbo_set_threads": (None, [C.c_int]),
     }
@@ -156,6 +157,36 @@ class Oracle:
         LIB.lbo_set_particles(self._h, C.byref(self.unit), _d(self.origin), lst.n,
                               tag.ctypes.data_as(C.POINTER(C.c_int)), _d(lst.x), _d(lst.v),
                               _d(lst.omega), _d(lst.radius))
+
+    def set_rank(self, ntotal, pcoord):
+        """Make this lattice one rank's sub-domain of a decomposed run
+        (lbo_set_rank); its halos are then the caller's to fill."""
+        LIB.lbo_set_rank(self._h, (C.c_int * 3)(*[int(v) for v in ntotal]),
+                         (C.c_int * 3)(*[int(v) for v in pcoord]))
+
+    def halo_exchange(self):
+        LIB.lbo_halo_exchange(self._h)
+
+    def collide_stream(self):
+        LIB.lbo_collide_stream(self._h)
+
+    def hydro(self, lst):
+        """Force, torque and stresslet of every list entry over this
+        lattice's interior nodes (calculateHydrodynamicInteractions)."""
+        n = lst.n
+        F, T, S = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 6))
+        x = np.ascontiguousarray(lst.x, dtype=np.float64)
+        for p in range(n):
+            LIB.lbo_hydro(self._h, C.byref(self.unit), _d(self.origin), int(lst.tag[p]),
+                          _d(x[p]), float(lst.radius[p]), _d(F[p]), _d(T[p]), _d(S[p]))
+        return F, T, S
+
+    def f_view(self):
+        """The current (post-stream) populations with halos, in place (the
+        buffer changes at every collide_stream)."""
+        ptr = LIB.lbo_f(self._h)
+        return np.ctypeslib.as_array(ptr, shape=(self.nodes_h * self.q,)).reshape(*self.nh,
+                                                                                   self.q)
 
     def zou_he_face(self, kind, plane, a, b=(0, 0), u=(0.0, 0.0, 0.0), rho=1.0):
         """One face function on plane x [a0,a1] x [b0,b1] (reference halo'd

@@ -68,13 +68,15 @@ def test_stokes_drag_deck():
     err = abs(abs(Fo[0, 0]) - stokes) / stokes * 100.0
     ref = GOLD["stokes_drag_tau0p7"]["force_error_percent"]
     print(f"  drag error {err:.6f} % (reference {ref:.6f} %)", flush=True)
-    # Measured: 0.9627 % against the published 0.6512 %.  The drag is still
-    # drifting slowly at step 100000 (a long transient of the stale-halo y
-    # faces), and the library equals the one-rank oracle bit for bit on this
-    # setting (test_moving_sphere_wall_axis_without_bc_long), so the 0.31
-    # point gap lies between the one-rank restatement and the published
-    # 8-rank run, which cannot be rerun here.  The bound below is the
-    # tolerance of that open difference, not of the library.
+    # Measured: 0.9627 % against the published 0.6512 %.  The library equals
+    # the one-rank oracle bit for bit on this setting
+    # (test_moving_sphere_wall_axis_without_bc_long), and the deck's
+    # `processors 2 2 2` decomposition, emulated in the oracle, equals one
+    # rank bit for bit (test_oracle_ranks.py), so the published run's ranks
+    # are not the cause either.  The other decks of the same code (torque,
+    # free rotation, 2D drag) reproduce their published numbers; what
+    # separates this one stays open.  The bound is the tolerance of that open
+    # difference, not of the library.
     assert abs(err - ref) < 0.5
     assert err < 1.5
     lat.close()
