@@ -8,7 +8,7 @@
   velocity inlet U = 0.005 at x = 0 and density outlet, periodic y/z,
   gravity Fx = -2e-6, 20000 spheres d = 12 moving linearly, tau 0.7, FP64
   and FP32 storage for 10 000 steps: FP32 drag/torque within 1e-5 of FP64
-  (measured 1.0e-6 / 1.4e-6), and the step rate of both forms.  The spheres are seeded over the whole
+  (measured 8.1e-7 / 9.3e-7), and the step rate of both forms.  The spheres are seeded over the whole
   column (phi = 0.27): 20000 of them in its lower third would need
   phi = 0.81, beyond random sequential addition.
 * D3Q19 256^3 periodic, Fx = 5e-6, 30 % volume fraction of fixed spheres with
@@ -110,8 +110,8 @@ def test_fluidised_bed_10000_steps_fp32_vs_fp64():
     in FP64 and FP32 storage, 20 000 spheres d = 12 moving linearly (new
     positions mapped every step), xFlow inlet U = 0.005 / outlet rho = 1,
     gravity Fx = -2e-6, tau 0.7.  The FP32 drag and torque of every sphere
-    within 1e-5 of FP64 (relative to the largest; measured 1.0e-6 and
-    1.4e-6), the step rate of both.  The pressure drop over the bed holds
+    within 1e-5 of FP64 (relative to the largest; measured 8.1e-7 and
+    9.3e-7), the step rate of both.  The pressure drop over the bed holds
     the density 0.1 above the stored shift rho0 = 1 at the bottom, so the
     FP32 collision must keep mass exactly at drho != 0: with the nearest
     floats as lattice weights (summing to 1 + 1.5e-8) the column gains a
