@@ -694,7 +694,7 @@ __device__ __forceinline__ float eq_odd(float Ax, float Ay, float Az) {
   }
 }
 
-template <int DIM, int Q, typename Emit>
+template <int DIM, int Q, bool EXACT, typename Emit>
 __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f, unsigned n,
                                                   const SlotsF32& S, Emit&& emit);
 
@@ -723,15 +723,18 @@ template <int DIM, int Q, int ST>
 __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, unsigned rb,
                                              unsigned uc, const SlotsF32& S) {
   const int64_t plane = a.g.plane;
-  collide_lowp_node<DIM, Q>(a, f, rb + uc, S, [&](auto qc, float out) {
+  collide_lowp_node<DIM, Q, ST == ST_F32>(a, f, rb + uc, S, [&](auto qc, float out) {
     st<ST>(pop_ptr_w<ST, Q>(a.fout, decltype(qc)::value, plane, rb, uc), out, a.scale);
   });
 }
 
 // One node in FP32: f holds the node's post-stream populations (after the
 // faces); emit(q, value) receives each post-collision population in q order
-// and the node's force goes to Fh.
-template <int DIM, int Q, typename Emit>
+// and the node's force goes to Fh.  EXACT (FP32 storage) forms the even
+// terms with w drho unrounded (lowp_even); FP16 storage, whose own rounding
+// is 2^13 times coarser, keeps the plain form w (drho + rho t), which needs
+// fewer registers.
+template <int DIM, int Q, bool EXACT, typename Emit>
 __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f, unsigned n,
                                                   const SlotsF32& S, Emit&& emit) {
 #pragma clang fp contract(fast)
@@ -769,8 +772,10 @@ __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f
       Unroll<0, Q>::run([&]<int q>() {
         constexpr float w = lowp_w<Q, q>();
         const float eu = edot<Q, q>(ux, uy, uz);
-        const float geq = lowp_even(w, drho, lowp_rw<Q, q>(rw), 4.5f * eu * eu - usq,
-                                    eq_odd<Q, q>(Ax, Ay, Az));
+        const float geq =
+            EXACT ? lowp_even(w, drho, lowp_rw<Q, q>(rw), 4.5f * eu * eu - usq,
+                              eq_odd<Q, q>(Ax, Ay, Az))
+                  : w * (drho + rho * (4.5f * eu * eu - usq)) + eq_odd<Q, q>(Ax, Ay, Az);
         float out = f[q] + omega * (geq - f[q]);
         if (a.guo) {
           constexpr float ex = VelSet<Q>::ex[q], ey = VelSet<Q>::ey[q], ez = VelSet<Q>::ez[q];
@@ -804,8 +809,10 @@ __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f
           constexpr int cx = VelSet<Q>::ex[q], cy = VelSet<Q>::ey[q], cz = VelSet<Q>::ez[q];
           const float gi = f[q];
           const float eu = edot<Q, q>(ux, uy, uz);
-          const float geq = lowp_even(w, drho, lowp_rw<Q, q>(rw), 4.5f * eu * eu - usq,
-                                    eq_odd<Q, q>(Ax, Ay, Az));
+          const float geq =
+              EXACT ? lowp_even(w, drho, lowp_rw<Q, q>(rw), 4.5f * eu * eu - usq,
+                                eq_odd<Q, q>(Ax, Ay, Az))
+                    : w * (drho + rho * (4.5f * eu * eu - usq)) + eq_odd<Q, q>(Ax, Ay, Az);
           float sc1 = 0.f, sc2 = 0.f;
           // g_s - g_i of the solid equilibrium in one rounding (the addend
           // -g_i keeps it unbiased for a particle at rest too); its momentum
@@ -813,15 +820,19 @@ __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f
           // node's exchange B rho (u_s - u) for particle speeds up to the fluid's
           if (o1) {
             const float e1 = edot<Q, q>(us1[0], us1[1], us1[2]);
-            const float ds = lowp_even(w, drho, lowp_rw<Q, q>(rw),
-                                       3.0f * e1 + 4.5f * e1 * e1 - u1sq, -gi);
+            const float ds =
+                EXACT ? lowp_even(w, drho, lowp_rw<Q, q>(rw), 3.0f * e1 + 4.5f * e1 * e1 - u1sq,
+                                  -gi)
+                      : w * (drho + rho * (3.0f * e1 + 4.5f * e1 * e1 - u1sq)) - gi;
             sc1 = ds + omo * (gi - geq);
           }
           if constexpr (TWO) {
             if (o2) {
               const float e2 = edot<Q, q>(us2[0], us2[1], us2[2]);
-              const float ds = lowp_even(w, drho, lowp_rw<Q, q>(rw),
-                                         3.0f * e2 + 4.5f * e2 * e2 - u2sq, -gi);
+              const float ds =
+                  EXACT ? lowp_even(w, drho, lowp_rw<Q, q>(rw), 3.0f * e2 + 4.5f * e2 * e2 - u2sq,
+                                    -gi)
+                        : w * (drho + rho * (3.0f * e2 + 4.5f * e2 * e2 - u2sq)) - gi;
               sc2 = ds + omo * (gi - geq);
             }
           }
