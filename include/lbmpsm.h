@@ -165,7 +165,7 @@ int lbmpsm_step(lbmpsm_ctx* ctx, int flags);
  * results go to `out`, caller device memory of [n][12] doubles per list
  * entry: F (3), T (3), S (6: xx yy zz xy xz yz), physical units, n the
  * length of the list last given to lbmpsm_set_particles(_device).
- * Limits and errors: at most 43690 list entries (LBMPSM_ERR_LIMIT beyond);
+ * Limits and errors: at most 87381 list entries (LBMPSM_ERR_LIMIT beyond);
  * LBMPSM_STEP_STORE_MACROS only after an lbmpsm_step with that flag has
  * allocated the macro arrays (LBMPSM_ERR_STATE otherwise); no profiling.
  * The static array is shared by every call on one device: calls that

@@ -75,7 +75,7 @@ struct CollideArgs {
 
 // Workgroups per entry of the force/torque/stresslet reduction.
 #ifndef LBMPSM_HYDRO_GROUPS
-#define LBMPSM_HYDRO_GROUPS 4
+#define LBMPSM_HYDRO_GROUPS 2
 #endif
 constexpr int kHydroGroups = LBMPSM_HYDRO_GROUPS;
 

@@ -751,7 +751,7 @@ __global__ __launch_bounds__(256) void k_map_general(const MapArgs a) {
 #define LBMPSM_HYDRO_THREADS 256
 #endif
 #ifndef LBMPSM_HYDRO_U
-#define LBMPSM_HYDRO_U 3
+#define LBMPSM_HYDRO_U 8
 #endif
 constexpr int kHydroThreads = LBMPSM_HYDRO_THREADS;
 constexpr int kHydroU = LBMPSM_HYDRO_U;
