@@ -124,6 +124,17 @@ namespace {
 #endif
 constexpr int kSfPlacement = LBMPSM_SF_PLACEMENT;
 
+// 3D map bricks (x, y, z nodes; 256 nodes, one per thread of a workgroup).
+#ifndef LBMPSM_BRICK3_I
+#define LBMPSM_BRICK3_I 4
+#endif
+#ifndef LBMPSM_BRICK3_J
+#define LBMPSM_BRICK3_J 4
+#endif
+constexpr int kBrick3[3] = {LBMPSM_BRICK3_I, LBMPSM_BRICK3_J,
+                            256 / (LBMPSM_BRICK3_I * LBMPSM_BRICK3_J)};
+static_assert(kBrick3[0] * kBrick3[1] * kBrick3[2] == 256, "256-node bricks");
+
 int fail(lbmpsm_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   else g_create_error = msg;
@@ -828,7 +839,7 @@ int lbmpsm_create(const lbmpsm_config* cfg, lbmpsm_ctx** out) {
   // node bricks for the particle map: 256 nodes each
   BrickDev& B = c->br;
   if (g.dim == 3) {
-    B.bi = 4; B.bj = 4; B.bk = 16;
+    B.bi = kBrick3[0]; B.bj = kBrick3[1]; B.bk = kBrick3[2];
   } else {
     B.bi = 16; B.bj = 16; B.bk = 1;
   }
