@@ -291,6 +291,13 @@ CollideArgs make_args(lbmpsm_ctx* c, int store_macros) {
   a.scale = c->scale;
   a.inv_scale = c->inv_scale;
   a.cb = c->d_cb;
+  a.cbf = (const float*)(c->d_cb + 19);
+  {
+    const double k = 3.0 * (c->g.q == 19 ? VelSet<19>::w[1] : VelSet<9>::w[1]);
+    const double v[9] = {a.omega, a.rho0, F[0], F[1], F[2], 0.5 * k * F[0], 0.5 * k * F[1],
+                         0.5 * k * F[2], a.guo_pref};
+    for (int i = 0; i < 9; ++i) a.lowp_c[i] = (float)v[i];
+  }
   a.faces = c->d_faces;
   a.nfaces = c->nfaces;
   return a;
@@ -834,7 +841,7 @@ int lbmpsm_create(const lbmpsm_config* cfg, lbmpsm_ctx** out) {
     c->slots.usf = reinterpret_cast<float*>(c->slots.us);
     c->slots.Fhf = reinterpret_cast<float*>(c->slots.Fh);
   }
-  rc |= dev_alloc(c, &c->d_cb, 19);
+  rc |= dev_alloc(c, &c->d_cb, 19 + 10);  // the doubles, then the same as floats
   rc |= dev_alloc(c, &c->d_faces, kMaxFaces);
   // node bricks for the particle map: 256 nodes each
   BrickDev& B = c->br;
@@ -951,7 +958,10 @@ int lbmpsm_init_equilibrium(lbmpsm_ctx* c, double rho, double ux, double uy, dou
                                                         : c->cb_raw[q] - wq(Q, q) * c->rho0;
   }
   c->cur = 0;
+  float cbf[19];
+  for (int q = 0; q < 19; ++q) cbf[q] = (float)c->cb_store[q];
   HIPCHK(c, hipMemcpyAsync(c->d_cb, c->cb_store, sizeof(c->cb_store), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_cb + 19, cbf, sizeof(cbf), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, launch_fill_planes(c->cfg.storage, c->f[0], c->g, c->cb_store, c->rho0, c->scale, c->stream));
   HIPCHK(c, launch_slots_reset(c->g, c->slots, c->stream));

@@ -299,7 +299,9 @@ __device__ __forceinline__ void gather_row(const CollideArgs& a, const Row& r, i
     // both values unconditionally (src stays inside the buffer: nbr keeps
     // the coordinate off a non-periodic edge), then a select: with no branch
     // around the load, the loads of all directions are in flight together
-    const C cbv = (C)a.cb[q];
+    C cbv;
+    if constexpr (ST == ST_F64) cbv = a.cb[q];
+    else cbv = a.cbf[q];
     const C v = ld<ST>(src, a.inv_scale);
     f[q] = (rok && cok) ? v : cbv;
   });
@@ -715,7 +717,7 @@ __global__ __launch_bounds__(256) LBMPSM_LOWP_OCC void k_collide_lowp(const Coll
   SlotsF32 S;
   load_slots_f32(a, rb + (unsigned)c, m, S);
   if (__any(face)) apply_faces<Q, float>(f, a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c,
-                                         DIM == 3 ? c : 0, (float)a.rho0);
+                                         DIM == 3 ? c : 0, a.lowp_c[1]);
   collide_lowp<DIM, Q, ST>(a, f, rb, (unsigned)c, S);
 }
 
@@ -740,12 +742,11 @@ __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f
 #pragma clang fp contract(fast)
   const Geom& g = a.g;
   const int64_t plane = g.plane;
-  const float omega = (float)a.omega;
-  const float rho0 = (float)a.rho0;
-  const float F0 = (float)a.F[0], F1 = (float)a.F[1], F2 = (float)a.F[2];
-  const float hF0 = (float)(0.5 * OddK<Q>::k * a.F[0]), hF1 = (float)(0.5 * OddK<Q>::k * a.F[1]),
-              hF2 = (float)(0.5 * OddK<Q>::k * a.F[2]);
-  const float gp = (float)a.guo_pref;
+  const float omega = a.lowp_c[0];
+  const float rho0 = a.lowp_c[1];
+  const float F0 = a.lowp_c[2], F1 = a.lowp_c[3], F2 = a.lowp_c[4];
+  const float hF0 = a.lowp_c[5], hF1 = a.lowp_c[6], hF2 = a.lowp_c[7];
+  const float gp = a.lowp_c[8];
   {
     float drho, jx, jy, jz;
     moments<Q, float>(f, drho, jx, jy, jz);

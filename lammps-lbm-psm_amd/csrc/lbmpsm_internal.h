@@ -66,9 +66,14 @@ struct CollideArgs {
   double F[3];
   double invCs2, invCs4;
   double rho0;          // reference density of shifted storage
+  // the FP32 form's per-launch constants, rounded once on the host: omega,
+  // rho0, F[3], 3 w_axis F[3] / 2 (the half force of the odd equilibrium),
+  // 1 - omega/2
+  float lowp_c[9];
   float scale, inv_scale;  // FP16 storage scaling (power of two)
   const double* cb;     // [q] value of populations pulled from beyond a non-periodic
                         // halo, in storage form (device memory)
+  const float* cbf;     // the same rounded to float, for the FP32 form
   const Face* faces;    // [nfaces] Zou-He faces in application order (device memory)
   int nfaces;
 };
