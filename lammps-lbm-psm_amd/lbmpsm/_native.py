@@ -85,7 +85,7 @@ class LbmPsmError(RuntimeError):
 def _load() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise ImportError(
-            f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C lammps-lbm-psm_amd)"
+            f"{LIB_PATH} is not built; run make -C lammps-lbm-psm_amd"
         )
     lib = C.CDLL(LIB_PATH)
     P = C.c_void_p
