@@ -140,3 +140,52 @@ def test_fp32_mass_kept_at_raised_density(dim):
     lat.close()
     print(f"  mass drift per node {drift:.3e}", flush=True)
     assert drift < 2e-8
+
+
+def run_2d(storage, steps=1500):
+    """D2Q9 channel 240 x 120, `boundary f p`, `lbm-psm-bc xFlow` (Zou-He
+    inlet and density outlet in x, velocity faces on the y planes), tau 0.7,
+    body force along x, 10 moving disks of radius 5 to 8."""
+    n = (240, 120, 1)
+    per = (0, 1, 1)
+    box = lattice_box(2, n, per)
+    atoms = random_atoms(box, [5.0, 6.0, 7.0, 8.0, 5.5, 6.5, 7.5, 5.0, 6.0, 7.0], seed=41,
+                         vmax=2e-3, wmax=2e-4, gap=2.0)
+    lat = lbmpsm.Lattice(lbmpsm.lattice_units_config(2, n, per, 0.7, storage=storage,
+                                                     F_lbm=(2e-6, 0, 0), u_lb=0.01))
+    lat.set_bc("xFlow")
+    for s in range(steps):
+        H.move_linear(atoms, box, s, 1.0)
+        lst = H.build_list(atoms, box, 12.0)
+        lat.set_particles(lst.tag, lst.x, lst.v, lst.omega, lst.radius)
+        lat.step(store_macros=(s == steps - 1))
+    F, T, S = lat.hydro()
+    Fo, To = H.reduce_to_owners(lst, F, T)
+    _, u = lat.macros()
+    m = lat.mass()
+    lat.close()
+    return Fo, To, u, m
+
+
+@pytest.fixture(scope="module")
+def ref64_2d():
+    print("\n[case] precision reference FP64 D2Q9 240x120 xFlow, 10 moving disks, 1500 steps",
+          flush=True)
+    return run_2d("f64")
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f16", 2e-3)])
+def test_reduced_storage_2d_channel(ref64_2d, storage, tol):
+    """The D2Q9 forms of the reduced-storage collide (fluid and particle
+    waves, the 2D Zou-He faces on shifted populations) against FP64: disk
+    drag and torque and the velocity field within tol of their largest
+    values (bars as in 3D: 1e-5 for FP32, 2e-3 for FP16)."""
+    print(f"\n[case] precision D2Q9 {storage}", flush=True)
+    F0, T0, u0, m0 = ref64_2d
+    F, T, u, m = run_2d(storage)
+    eF = np.max(np.linalg.norm(F - F0, axis=1)) / np.max(np.linalg.norm(F0, axis=1))
+    eT = np.max(np.abs(T[:, 2] - T0[:, 2])) / np.max(np.abs(T0[:, 2]))
+    eu = np.max(np.abs(u - u0)) / np.max(np.abs(u0))
+    print(f"  {storage}: drag {eF:.2e} torque {eT:.2e} velocity {eu:.2e} "
+          f"mass {abs(m - m0) / m0:.2e}", flush=True)
+    assert eF <= tol and eT <= tol and eu <= tol
