@@ -630,8 +630,18 @@ __device__ __forceinline__ void map_brick(const MapArgs& a, int64_t b, MapEntry*
               const int nj = z[3] - z[2], nk = z[5] - z[4];
               if (zone_coded(a, z[1] - z[0], nj, nk)) {
                 coded |= 1u << u;
-                const int idx = ((i - z[0]) * nj + (j - z[2])) * nk + (k - z[4]);
-                cd[u] = a.sfcode[(size_t)raw[sidx] * a.zone_cap + idx];
+                // the reference's early exits decide most cells (and give
+                // the codes the solid-fraction kernel stored for them); only
+                // a cell that needs its count loads the code
+                const double* xl = ent[sidx].xl;
+                const int cls = sf_classify(a.hsq_half, (double)ri - xl[0], (double)rj - xl[1],
+                                            (double)rk - xl[2], xl[3]);
+                if (cls == SF_ONE) {
+                  cd[u] = kSfCodeOne;
+                } else if (cls == SF_COUNT) {
+                  const int idx = ((i - z[0]) * nj + (j - z[2])) * nk + (k - z[4]);
+                  cd[u] = a.sfcode[(size_t)raw[sidx] * a.zone_cap + idx];
+                }
               }
             }
           }
