@@ -238,6 +238,47 @@ def cpu_baseline(wl, lists, steps_all=2):
                                    f"1 thread"}}
 
 
+def reference_benchmark_deck():
+    """The reference's own published benchmark (benchmark/in.lbmpsm_1:8-40):
+    a fixed sphere of d = 0.0005 m at the centre of a 0.005 m cube,
+    `boundary p f p`, `fix lbm-psm ... Nlc 31 lc 0.0005 rho 1000 nu 1e-4 Re 0.1
+    tau 0.7` (300 x 301 x 300 = 27,090,000 nodes), `lbm-psm-bc shear`, 100
+    steps, driven as the fix drives it (host list, step, F/T/S download every
+    step).  The published quantity is LAMMPS's total wall time on a 2 x Xeon
+    Silver 4114 workstation (benchmark/benchmark-results.dat:2-7, copied into
+    tests/golden/reference_verification.json); here the lattice set-up and
+    the 100 steps are timed on one GPU, FP64, without the deck's VTI output."""
+    import lbmpsm
+    from lbmpsm import harness as H
+    box_l, r = 0.005, 0.00025
+    cfg = lbmpsm.config_from_fix(3, 31, 0.0005, 1000.0, 1e-4, 0.1, 0.7, (0.0, 0.0, 0.0),
+                                 (0.0, 0.0, 0.0), (box_l, box_l, box_l), (1, 0, 1))
+    box = H.Box(lo=np.zeros(3), prd=np.array([box_l] * 3), periodic=(1, 0, 1), dim=3)
+    atoms = H.Atoms(tag=np.array([1], np.int32), x=np.array([[0.0025] * 3]), v=np.zeros((1, 3)),
+                    omega=np.zeros((1, 3)), radius=np.array([r]))
+    lst = H.build_list(atoms, box, cutoff=0.001)
+    t0 = time.perf_counter()
+    lat = lbmpsm.Lattice(cfg)
+    lat.set_bc("shear")
+    lat.sync()
+    t1 = time.perf_counter()
+    for _ in range(100):
+        lat.set_particles(lst.tag, lst.x, lst.v, lst.omega, lst.radius)
+        lat.step()
+        lat.hydro()
+    t2 = time.perf_counter()
+    nodes = int(np.prod(lat.shape))
+    lat.close()
+    with open(os.path.join(ROOT, "tests", "golden", "reference_verification.json")) as fh:
+        pub = json.load(fh)["benchmark_wall_time_s"]
+    return {"deck": "benchmark/in.lbmpsm_1 (one fixed sphere in shear, Nlc 31, 100 steps)",
+            "nodes": nodes, "setup_s": round(t1 - t0, 3), "steps_s": round(t2 - t1, 3),
+            "MLUPS_steps": round(nodes * 100 / (t2 - t1) / 1e6, 1),
+            "published_wall_s": {f"{int(c)} cores": w for c, w in pub["rows"]},
+            "published_source": pub["source"],
+            "note": "published: LAMMPS total wall time incl. set-up and VTI output, CPU"}
+
+
 def measure(wl, storages, warmup, steps, host=True):
     lists = wl.lists(warmup + steps)
     staged = stage_lists(lists)
@@ -314,6 +355,8 @@ def main():
             secondary[st] = {"MLUPS": round(other.nodes * args.steps / r["el"] / 1e6, 1),
                              "ms_per_step": round(r["el"] / args.steps * 1e3, 4),
                              "roofline": roofline(other, st, r)}
+        print("[bench] the reference's benchmark deck", file=sys.stderr, flush=True)
+        secondary["reference_benchmark"] = reference_benchmark_deck()
 
     cpu = None
     if not args.no_cpu_baseline:
