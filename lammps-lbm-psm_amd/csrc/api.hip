@@ -661,6 +661,7 @@ int step_launch(lbmpsm_ctx* c, int store, double* part, double* out, bool own_ou
   h.xl = c->part.xl;
   h.rzone = c->part.rzone;
   h.slot_tag = c->slots.tag;
+  h.mask = c->slots.mask;
   h.Fh = c->slots.Fh;
   h.Fhf = c->slots.Fhf;
   h.forceFactor = c->cfg.force_factor;

@@ -227,6 +227,7 @@ struct HydroArgs {
   const double* xl;
   const int* rzone;
   const int* slot_tag;
+  const uint8_t* mask;  // slot occupancy: a slot without its bit holds tag 0
   const double* Fh;
   const float* Fhf;  // node forces in float (FP32 / FP16 storage), else null
   double forceFactor, torqueFactor;

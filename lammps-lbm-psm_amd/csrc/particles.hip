@@ -800,9 +800,19 @@ __global__ __launch_bounds__(kHydroThreads) void k_hydro(const HydroArgs a, cons
       for (int u = 0; u < U; ++u) {
         sl[u] = -1;
         nn[u] = 0;
+        if (i0 + u < phi) nn[u] = (z[0] + i0 + u) * psz + nb;
+      }
+      // the occupancy bits first: a slot without its bit holds tag 0, so
+      // only the tags of occupied slots are loaded (most nodes hold one
+      // particle at most, and the zone's corners none)
+      uint8_t mk[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) mk[u] = i0 + u < phi ? a.mask[nn[u]] : (uint8_t)0;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
         if (i0 + u < phi) {
-          nn[u] = (z[0] + i0 + u) * psz + nb;
-          const int a0 = a.slot_tag[nn[u]], a1 = a.slot_tag[plane + nn[u]];
+          const int a0 = (mk[u] & kMaskOcc0) ? a.slot_tag[nn[u]] : 0;
+          const int a1 = (mk[u] & kMaskOcc1) ? a.slot_tag[plane + nn[u]] : 0;
           sl[u] = tag == a0 ? 0 : (tag == a1 ? 1 : -1);
         }
       }
