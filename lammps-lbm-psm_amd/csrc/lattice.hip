@@ -395,15 +395,40 @@ __device__ __forceinline__ void load_slots_f64(const CollideArgs& a, unsigned n,
 }
 
 // Collision of one node whose post-stream populations are in f[] (FP64).
-template <int DIM, int Q>
+template <int DIM, int Q, int FAX>
 __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, unsigned rb,
                                             unsigned uc, SlotsF64& S);
+
+// Guo's forcing term of direction q without its prefactor (1 - omega/2),
+// src/LBM_PSM_BGK_dynamics.h:40-114: w ((e - u).F / cs^2 + (e.u)(e.F) / cs^4),
+// in the reference's operand order.  FAX >= 0: only F[FAX] is nonzero (a
+// body force along one lattice axis, as gravity usually is); the terms of
+// the other components are then exact zeros, and leaving them out of the
+// sums changes a result at most in the sign of a zero.
+template <int Q, int q, int FAX>
+__device__ __forceinline__ double guo_f64(double ux, double uy, double uz, double eu, double F0,
+                                          double F1, double F2, double inv2, double inv4) {
+  constexpr double w = VelSet<Q>::w[q];
+  constexpr int cx = VelSet<Q>::ex[q], cy = VelSet<Q>::ey[q], cz = VelSet<Q>::ez[q];
+  constexpr double ex = cx, ey = cy, ez = cz;
+  if constexpr (FAX < 0) {
+    const double eF = edot<Q, q>(F0, F1, F2);
+    return w * (((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) * inv2 + (eu * inv4 * eF));
+  } else {
+    constexpr int ca = FAX == 0 ? cx : (FAX == 1 ? cy : cz);
+    constexpr double ea = ca;
+    const double Fa = FAX == 0 ? F0 : (FAX == 1 ? F1 : F2);
+    const double ua = FAX == 0 ? ux : (FAX == 1 ? uy : uz);
+    if constexpr (ca != 0) return w * ((ea - ua) * Fa * inv2 + (eu * inv4 * eterm<ca>(Fa)));
+    else return w * ((ea - ua) * Fa * inv2);
+  }
+}
 
 // Bulk: every stored node, one wave per 64-node chunk of a row.  A node on
 // Zou-He faces has them applied, in table order, to its pulled populations;
 // its pulls are part of the row's coalesced loads (a z face node would
 // otherwise be a strided access of its own).
-template <int DIM, int Q, bool POST>
+template <int DIM, int Q, bool POST, int FAX>
 __global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_f64(const CollideArgs a) {
   const Geom& g = a.g;
   Row r;
@@ -422,10 +447,10 @@ __global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_f64(const CollideArg
   load_slots_f64(a, rb + (unsigned)c, m, S);
   if (__any(face)) apply_faces<Q, double>(f, a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c,
                                           DIM == 3 ? c : 0, 0.0);
-  collide_f64<DIM, Q>(a, f, rb, (unsigned)c, S);
+  collide_f64<DIM, Q, FAX>(a, f, rb, (unsigned)c, S);
 }
 
-template <int DIM, int Q>
+template <int DIM, int Q, int FAX>
 __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, unsigned rb,
                                             unsigned uc, SlotsF64& S) {
   const Geom& g = a.g;
@@ -466,13 +491,8 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
         const double eu = edot<Q, q>(ux, uy, uz);
         const double feq = rho * w * (1.0 + eu * inv2 + eu * eu * 0.5 * inv4 - usqt);
         double out = f[q] + (feq - f[q]) * omega;
-        if (a.guo) {
-          constexpr double ex = VelSet<Q>::ex[q], ey = VelSet<Q>::ey[q], ez = VelSet<Q>::ez[q];
-          const double eF = edot<Q, q>(F0, F1, F2);
-          const double Fi =
-              w * (((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) * inv2 + (eu * inv4 * eF));
-          out = out + a.guo_pref * Fi;
-        }
+        if (FAX >= 0 || a.guo)
+          out = out + a.guo_pref * guo_f64<Q, q, FAX>(ux, uy, uz, eu, F0, F1, F2, inv2, inv4);
         st<ST_F64>(pop_ptr_w<ST_F64, Q>(a.fout, q, plane, rb, uc), out, 1.0f);
       });
     } else {
@@ -515,13 +535,8 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
           double out = fi + (1.0 - Btot) * (feq - fi) * omega + B1 * sc1;
           if constexpr (TWO) out = out + B2 * sc2;
           else out = out + 0.0;
-          if (a.guo) {
-            constexpr double ex = cx, ey = cy, ez = cz;
-            const double eF = edot<Q, q>(F0, F1, F2);
-            const double Fi =
-                w * (((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) * inv2 + (eu * inv4 * eF));
-            out = out + a.guo_pref * Fi;
-          }
+          if (FAX >= 0 || a.guo)
+            out = out + a.guo_pref * guo_f64<Q, q, FAX>(ux, uy, uz, eu, F0, F1, F2, inv2, inv4);
           st<ST_F64>(pop_ptr_w<ST_F64, Q>(a.fout, q, plane, rb, uc), out, 1.0f);
           // Node force on each slot, src/LBM_PSM_BGK_dynamics.h:101-113.
           if (o1) {
@@ -1054,10 +1069,22 @@ static hipError_t collide_dim(int storage, const CollideArgs& a, hipStream_t s) 
   const int wpb = LBMPSM_ROW_WAVES > 0 ? LBMPSM_ROW_WAVES : std::min((ncols + 63) / 64, 4);
   dim3 grid((unsigned)((waves + wpb - 1) / wpb)), block((unsigned)(64 * wpb));
   switch (storage) {
-    case ST_F64:
-      if (a.poststream) hipLaunchKernelGGL((k_collide_f64<DIM, Q, true>), grid, block, 0, s, a);
-      else hipLaunchKernelGGL((k_collide_f64<DIM, Q, false>), grid, block, 0, s, a);
+    case ST_F64: {
+      // a body force along one lattice axis takes the kernel formed for it
+      int fax = -1;
+      if (a.guo) {
+        const int nz = (a.F[0] != 0.0) + (a.F[1] != 0.0) + (a.F[2] != 0.0);
+        if (nz == 1) fax = a.F[0] != 0.0 ? 0 : (a.F[1] != 0.0 ? 1 : 2);
+        if (DIM == 2 && fax == 2) fax = -1;
+      }
+      if (a.poststream) hipLaunchKernelGGL((k_collide_f64<DIM, Q, true, -1>), grid, block, 0, s, a);
+      else if (fax == 0) hipLaunchKernelGGL((k_collide_f64<DIM, Q, false, 0>), grid, block, 0, s, a);
+      else if (fax == 1) hipLaunchKernelGGL((k_collide_f64<DIM, Q, false, 1>), grid, block, 0, s, a);
+      else if (DIM == 3 && fax == 2)
+        hipLaunchKernelGGL((k_collide_f64<DIM, Q, false, DIM == 3 ? 2 : -1>), grid, block, 0, s, a);
+      else hipLaunchKernelGGL((k_collide_f64<DIM, Q, false, -1>), grid, block, 0, s, a);
       break;
+    }
     case ST_F32:
       if (a.poststream) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32, true>), grid, block, 0, s, a);
       else hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32, false>), grid, block, 0, s, a);
