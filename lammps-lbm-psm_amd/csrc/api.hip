@@ -287,6 +287,7 @@ CollideArgs make_args(lbmpsm_ctx* c, int store_macros) {
   const double csPow4 = csPow2 * csPow2;
   a.invCs2 = 1.0 / csPow2;
   a.invCs4 = 1.0 / csPow4;
+  a.hInvCs4 = 0.5 * a.invCs4;
   a.rho0 = c->rho0;
   a.scale = c->scale;
   a.inv_scale = c->inv_scale;

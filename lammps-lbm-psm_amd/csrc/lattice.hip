@@ -456,6 +456,11 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
   const Geom& g = a.g;
   const int64_t plane = g.plane;
   const double omega = a.omega, inv2 = a.invCs2, inv4 = a.invCs4;
+  // The reference's quadratic term (e.u)(e.u) 0.5 invCs4 (fluid) or
+  // 0.5 (e.u)(e.u) invCs4 (solid) halves a product exactly, so it equals
+  // (e.u)(e.u) (0.5 invCs4) bit for bit (a difference could only arise
+  // below 1e-300, where the term vanishes next to 1.0 anyway).
+  const double hinv4 = a.hInvCs4;
   const double F0 = a.F[0], F1 = a.F[1], F2 = a.F[2];
   {
     const unsigned n = rb + uc;
@@ -489,7 +494,7 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
       Unroll<0, Q>::run([&]<int q>() {
         constexpr double w = VelSet<Q>::w[q];
         const double eu = edot<Q, q>(ux, uy, uz);
-        const double feq = rho * w * (1.0 + eu * inv2 + eu * eu * 0.5 * inv4 - usqt);
+        const double feq = rho * w * (1.0 + eu * inv2 + eu * eu * hinv4 - usqt);
         double out = f[q] + (feq - f[q]) * omega;
         if (FAX >= 0 || a.guo)
           out = out + a.guo_pref * guo_f64<Q, q, FAX>(ux, uy, uz, eu, F0, F1, F2, inv2, inv4);
@@ -509,32 +514,34 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
       // TWO: the second slot's terms are formed.  Without LBMPSM_F64_SPLIT
       // every particle wave forms them (zero on lanes without a second
       // particle); with it, a wave in which no lane holds a second particle
-      // adds the reference's B2 * solid_coll2 = +0.0 as a constant instead
-      // (x + 0.0 keeps every bit of x but the sign of a zero, as the
-      // reference's sum does).
+      // leaves out the reference's B2 * solid_coll2 = +0.0 (adding it could
+      // only change the sign of a zero).
       auto pass = [&]<bool TWO>() __attribute__((always_inline)) {
         Unroll<0, Q>::run([&]<int q>() {
           constexpr double w = VelSet<Q>::w[q];
           constexpr int cx = VelSet<Q>::ex[q], cy = VelSet<Q>::ey[q], cz = VelSet<Q>::ez[q];
           const double fi = f[q];
           const double eu = edot<Q, q>(ux, uy, uz);
-          const double feq = rho * w * (1.0 + eu * inv2 + eu * eu * 0.5 * inv4 - usqt);
+          const double feq = rho * w * (1.0 + eu * inv2 + eu * eu * hinv4 - usqt);
+          // fi - feq is exactly -(feq - fi) (IEEE subtraction is
+          // antisymmetric up to the sign of a zero), so the solid terms
+          // reuse the fluid term's difference
+          const double dneq = feq - fi;
           double sc1 = 0.0, sc2 = 0.0;
           if (o1) {
             const double e1 = edot<Q, q>(us1[0], us1[1], us1[2]);
-            const double fs = rho * w * (1.0 + e1 * inv2 + 0.5 * e1 * e1 * inv4 - u1sq);
-            sc1 = fs - fi + oneMinusOmega * (fi - feq);
+            const double fs = rho * w * (1.0 + e1 * inv2 + e1 * e1 * hinv4 - u1sq);
+            sc1 = fs - fi + oneMinusOmega * -dneq;
           }
           if constexpr (TWO) {
             if (o2) {
               const double e2 = edot<Q, q>(us2[0], us2[1], us2[2]);
-              const double fs = rho * w * (1.0 + e2 * inv2 + 0.5 * e2 * e2 * inv4 - u2sq);
-              sc2 = fs - fi + oneMinusOmega * (fi - feq);
+              const double fs = rho * w * (1.0 + e2 * inv2 + e2 * e2 * hinv4 - u2sq);
+              sc2 = fs - fi + oneMinusOmega * -dneq;
             }
           }
-          double out = fi + (1.0 - Btot) * (feq - fi) * omega + B1 * sc1;
+          double out = fi + (1.0 - Btot) * dneq * omega + B1 * sc1;
           if constexpr (TWO) out = out + B2 * sc2;
-          else out = out + 0.0;
           if (FAX >= 0 || a.guo)
             out = out + a.guo_pref * guo_f64<Q, q, FAX>(ux, uy, uz, eu, F0, F1, F2, inv2, inv4);
           st<ST_F64>(pop_ptr_w<ST_F64, Q>(a.fout, q, plane, rb, uc), out, 1.0f);

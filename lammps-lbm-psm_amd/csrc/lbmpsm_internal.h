@@ -65,6 +65,7 @@ struct CollideArgs {
   double guo_pref;      // 1 - 0.5*omega
   double F[3];
   double invCs2, invCs4;
+  double hInvCs4;       // 0.5 * invCs4 (exact)
   double rho0;          // reference density of shifted storage
   // the FP32 form's per-launch constants, rounded once on the host: omega,
   // rho0, F[3], 3 w_axis F[3] / 2 (the half force of the odd equilibrium),
