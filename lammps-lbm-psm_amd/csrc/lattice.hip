@@ -625,9 +625,31 @@ __device__ __forceinline__ void load_slots_f32(const CollideArgs& a, unsigned n,
   }
 }
 
-template <int DIM, int Q, int ST>
+template <int DIM, int Q, int ST, int FAX>
 __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, unsigned rb,
                                              unsigned uc, const SlotsF32& S);
+
+// The bracket of the FP32 Guo term, 3 (e - u).F + 9 (e.u)(e.F); FAX >= 0:
+// only F[FAX] is nonzero, and the other components' terms (exact zeros) are
+// left out.
+template <int Q, int q, int FAX>
+__device__ __forceinline__ float guo_lowp(float ux, float uy, float uz, float eu, float F0,
+                                          float F1, float F2) {
+#pragma clang fp contract(fast)
+  constexpr int cx = VelSet<Q>::ex[q], cy = VelSet<Q>::ey[q], cz = VelSet<Q>::ez[q];
+  constexpr float ex = cx, ey = cy, ez = cz;
+  if constexpr (FAX < 0) {
+    const float eF = edot<Q, q>(F0, F1, F2);
+    return 3.0f * ((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) + 9.0f * eu * eF;
+  } else {
+    constexpr int ca = FAX == 0 ? cx : (FAX == 1 ? cy : cz);
+    constexpr float ea = ca;
+    const float Fa = FAX == 0 ? F0 : (FAX == 1 ? F1 : F2);
+    const float ua = FAX == 0 ? ux : (FAX == 1 ? uy : uz);
+    if constexpr (ca != 0) return 3.0f * ((ea - ua) * Fa) + 9.0f * eu * eterm<ca>(Fa);
+    else return 3.0f * ((ea - ua) * Fa);
+  }
+}
 
 // Weights of the FP32 form's even terms (w (drho + rho (...)), the solid
 // equilibrium and the Guo term).  The nearest floats to 1/3, 1/18, 1/36 sum
@@ -718,11 +740,11 @@ __device__ __forceinline__ float eq_odd(float Ax, float Ay, float Az) {
   }
 }
 
-template <int DIM, int Q, bool EXACT, typename Emit>
+template <int DIM, int Q, bool EXACT, int FAX, typename Emit>
 __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f, unsigned n,
                                                   const SlotsF32& S, Emit&& emit);
 
-template <int DIM, int Q, int ST, bool POST>
+template <int DIM, int Q, int ST, bool POST, int FAX>
 __global__ __launch_bounds__(256) LBMPSM_LOWP_OCC void k_collide_lowp(const CollideArgs a) {
   const Geom& g = a.g;
   Row r;
@@ -740,14 +762,14 @@ __global__ __launch_bounds__(256) LBMPSM_LOWP_OCC void k_collide_lowp(const Coll
   load_slots_f32(a, rb + (unsigned)c, m, S);
   if (__any(face)) apply_faces<Q, float>(f, a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c,
                                          DIM == 3 ? c : 0, a.lowp_c[1]);
-  collide_lowp<DIM, Q, ST>(a, f, rb, (unsigned)c, S);
+  collide_lowp<DIM, Q, ST, FAX>(a, f, rb, (unsigned)c, S);
 }
 
-template <int DIM, int Q, int ST>
+template <int DIM, int Q, int ST, int FAX>
 __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, unsigned rb,
                                              unsigned uc, const SlotsF32& S) {
   const int64_t plane = a.g.plane;
-  collide_lowp_node<DIM, Q, ST == ST_F32>(a, f, rb + uc, S, [&](auto qc, float out) {
+  collide_lowp_node<DIM, Q, ST == ST_F32, FAX>(a, f, rb + uc, S, [&](auto qc, float out) {
     st<ST>(pop_ptr_w<ST, Q>(a.fout, decltype(qc)::value, plane, rb, uc), out, a.scale);
   });
 }
@@ -758,7 +780,7 @@ __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, uns
 // terms with w drho unrounded (lowp_even); FP16 storage, whose own rounding
 // is 2^13 times coarser, keeps the plain form w (drho + rho t), which needs
 // fewer registers.
-template <int DIM, int Q, bool EXACT, typename Emit>
+template <int DIM, int Q, bool EXACT, int FAX, typename Emit>
 __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f, unsigned n,
                                                   const SlotsF32& S, Emit&& emit) {
 #pragma clang fp contract(fast)
@@ -800,11 +822,7 @@ __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f
                               eq_odd<Q, q>(Ax, Ay, Az))
                   : w * (drho + rho * (4.5f * eu * eu - usq)) + eq_odd<Q, q>(Ax, Ay, Az);
         float out = f[q] + omega * (geq - f[q]);
-        if (a.guo) {
-          constexpr float ex = VelSet<Q>::ex[q], ey = VelSet<Q>::ey[q], ez = VelSet<Q>::ez[q];
-          const float eF = edot<Q, q>(F0, F1, F2);
-          out += gp * w * (3.0f * ((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) + 9.0f * eu * eF);
-        }
+        if (FAX >= 0 || a.guo) out += gp * w * guo_lowp<Q, q, FAX>(ux, uy, uz, eu, F0, F1, F2);
         emit(std::integral_constant<int, q>{}, out);
       });
     } else {
@@ -861,11 +879,7 @@ __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f
           }
           float out = gi + (1.0f - Btot) * omega * (geq - gi) + B1 * sc1;
           if constexpr (TWO) out = out + B2 * sc2;
-          if (a.guo) {
-            constexpr float ex = cx, ey = cy, ez = cz;
-            const float eF = edot<Q, q>(F0, F1, F2);
-            out += gp * w * (3.0f * ((ex - ux) * F0 + (ey - uy) * F1 + (ez - uz) * F2) + 9.0f * eu * eF);
-          }
+          if (FAX >= 0 || a.guo) out += gp * w * guo_lowp<Q, q, FAX>(ux, uy, uz, eu, F0, F1, F2);
           emit(std::integral_constant<int, q>{}, out);
           const float t1 = -B1 * sc1;
           if constexpr (cx != 0) Fa[0] += cx * t1;
@@ -1075,15 +1089,15 @@ static hipError_t collide_dim(int storage, const CollideArgs& a, hipStream_t s) 
   const int64_t waves = rows * ((ncols + 63) / 64);
   const int wpb = LBMPSM_ROW_WAVES > 0 ? LBMPSM_ROW_WAVES : std::min((ncols + 63) / 64, 4);
   dim3 grid((unsigned)((waves + wpb - 1) / wpb)), block((unsigned)(64 * wpb));
+  // a body force along one lattice axis takes the kernels formed for it
+  int fax = -1;
+  if (a.guo) {
+    const int nz = (a.F[0] != 0.0) + (a.F[1] != 0.0) + (a.F[2] != 0.0);
+    if (nz == 1) fax = a.F[0] != 0.0 ? 0 : (a.F[1] != 0.0 ? 1 : 2);
+    if (DIM == 2 && fax == 2) fax = -1;
+  }
   switch (storage) {
     case ST_F64: {
-      // a body force along one lattice axis takes the kernel formed for it
-      int fax = -1;
-      if (a.guo) {
-        const int nz = (a.F[0] != 0.0) + (a.F[1] != 0.0) + (a.F[2] != 0.0);
-        if (nz == 1) fax = a.F[0] != 0.0 ? 0 : (a.F[1] != 0.0 ? 1 : 2);
-        if (DIM == 2 && fax == 2) fax = -1;
-      }
       if (a.poststream) hipLaunchKernelGGL((k_collide_f64<DIM, Q, true, -1>), grid, block, 0, s, a);
       else if (fax == 0) hipLaunchKernelGGL((k_collide_f64<DIM, Q, false, 0>), grid, block, 0, s, a);
       else if (fax == 1) hipLaunchKernelGGL((k_collide_f64<DIM, Q, false, 1>), grid, block, 0, s, a);
@@ -1093,12 +1107,20 @@ static hipError_t collide_dim(int storage, const CollideArgs& a, hipStream_t s) 
       break;
     }
     case ST_F32:
-      if (a.poststream) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32, true>), grid, block, 0, s, a);
-      else hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32, false>), grid, block, 0, s, a);
+      if (a.poststream) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32, true, -1>), grid, block, 0, s, a);
+      else if (fax == 0) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32, false, 0>), grid, block, 0, s, a);
+      else if (fax == 1) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32, false, 1>), grid, block, 0, s, a);
+      else if (DIM == 3 && fax == 2)
+        hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32, false, DIM == 3 ? 2 : -1>), grid, block, 0, s, a);
+      else hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32, false, -1>), grid, block, 0, s, a);
       break;
     case ST_F16:
-      if (a.poststream) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16, true>), grid, block, 0, s, a);
-      else hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16, false>), grid, block, 0, s, a);
+      if (a.poststream) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16, true, -1>), grid, block, 0, s, a);
+      else if (fax == 0) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16, false, 0>), grid, block, 0, s, a);
+      else if (fax == 1) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16, false, 1>), grid, block, 0, s, a);
+      else if (DIM == 3 && fax == 2)
+        hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16, false, DIM == 3 ? 2 : -1>), grid, block, 0, s, a);
+      else hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16, false, -1>), grid, block, 0, s, a);
       break;
     default: return hipErrorInvalidValue;
   }
