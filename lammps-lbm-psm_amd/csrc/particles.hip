@@ -139,9 +139,16 @@ __global__ __launch_bounds__(64) void k_prep_bin(const PrepArgs a) {
 struct MapEntry {
   int zone[6];
   int tag;
+  int raw;    // list index (staged lists: entries in list order)
+  int coded;  // the zone has solid-fraction codes (zone_coded)
   double xl[4];
   double kin[6];
 };
+
+// z0 <= v < z1 for z0 <= z1, as one unsigned comparison
+__device__ __forceinline__ bool in_range(int v, int z0, int z1) {
+  return (unsigned)(v - z0) < (unsigned)(z1 - z0);
+}
 
 // One state-machine transition of a node for one entry whose zone contains
 // it, given the entry's solid fraction at the node,
@@ -552,7 +559,7 @@ constexpr int kMapBatch = LBMPSM_MAP_BATCH;
 
 template <bool GENERAL>
 __device__ __forceinline__ void map_brick(const MapArgs& a, int64_t b, MapEntry* ent, int* raw,
-                                          int* perm, int* flag) {
+                                          int* flag) {
   const int tid = threadIdx.x;
   // independent loads first: the brick's count and list, the node's slots
   const int m = a.count[b];
@@ -588,26 +595,27 @@ __device__ __forceinline__ void map_brick(const MapArgs& a, int64_t b, MapEntry*
   const int ri = i + g.off[0], rj = j + g.off[1], rk = k + g.off[2];
 
   if (listed) {
-    // stage every listed entry at once (list slot = staging slot) and rank
-    // the list, entries being distinct: rank = number of smaller entries
+    // stage every listed entry at once, in list order: the lanes load their
+    // entries, rank the list (entries are distinct: rank = number of smaller
+    // entries) and store each entry at its rank
+    MapEntry L;
     if (tid < m) {
       raw[tid] = rv;
-      MapEntry& E = ent[tid];
-      for (int c = 0; c < 6; ++c) E.zone[c] = a.zone[6 * rv + c];
-      E.tag = a.tag[rv];
-      for (int c = 0; c < 4; ++c) E.xl[c] = a.xl[4 * rv + c];
-      for (int c = 0; c < 6; ++c) E.kin[c] = a.kin[6 * rv + c];
+      for (int c = 0; c < 6; ++c) L.zone[c] = a.zone[6 * rv + c];
+      L.tag = a.tag[rv];
+      L.raw = rv;
+      for (int c = 0; c < 4; ++c) L.xl[c] = a.xl[4 * rv + c];
+      for (int c = 0; c < 6; ++c) L.kin[c] = a.kin[6 * rv + c];
     }
     if (!GENERAL && tid == 0) *flag = 0;
     __syncthreads();
     if (tid < m) {
       int r = 0;
       for (int u = 0; u < m; ++u) r += raw[u] < rv;
-      perm[r] = tid;
-      if (!GENERAL) {
-        const int* z = ent[tid].zone;
-        if (!zone_coded(a, z[1] - z[0], z[3] - z[2], z[5] - z[4])) *flag = 1;
-      }
+      const int* z = L.zone;
+      L.coded = zone_coded(a, z[1] - z[0], z[3] - z[2], z[5] - z[4]) ? 1 : 0;
+      if (!GENERAL && !L.coded) *flag = 1;
+      ent[r] = L;
     }
     __syncthreads();
     // an entry without codes: the general kernel takes the whole brick
@@ -623,24 +631,24 @@ __device__ __forceinline__ void map_brick(const MapArgs& a, int64_t b, MapEntry*
         for (int u = 0; u < kMapBatch; ++u) {
           cd[u] = 0;
           if (r0 + u < m) {
-            const int sidx = perm[r0 + u];
-            const int* z = ent[sidx].zone;
-            if (i >= z[0] && i < z[1] && j >= z[2] && j < z[3] && k >= z[4] && k < z[5]) {
+            const MapEntry& E = ent[r0 + u];
+            const int z0 = E.zone[0], z1 = E.zone[1], z2 = E.zone[2], z3 = E.zone[3];
+            const int z4 = E.zone[4], z5 = E.zone[5];
+            if (in_range(i, z0, z1) & in_range(j, z2, z3) & in_range(k, z4, z5)) {
               in |= 1u << u;
-              const int nj = z[3] - z[2], nk = z[5] - z[4];
-              if (zone_coded(a, z[1] - z[0], nj, nk)) {
+              if (E.coded) {
                 coded |= 1u << u;
                 // the reference's early exits decide most cells (and give
                 // the codes the solid-fraction kernel stored for them); only
                 // a cell that needs its count loads the code
-                const double* xl = ent[sidx].xl;
-                const int cls = sf_classify(a.hsq_half, (double)ri - xl[0], (double)rj - xl[1],
-                                            (double)rk - xl[2], xl[3]);
+                const int cls = sf_classify(a.hsq_half, (double)ri - E.xl[0],
+                                            (double)rj - E.xl[1], (double)rk - E.xl[2], E.xl[3]);
                 if (cls == SF_ONE) {
                   cd[u] = kSfCodeOne;
                 } else if (cls == SF_COUNT) {
-                  const int idx = ((i - z[0]) * nj + (j - z[2])) * nk + (k - z[4]);
-                  cd[u] = a.sfcode[(size_t)raw[sidx] * a.zone_cap + idx];
+                  const int nj = z3 - z2, nk = z5 - z4;
+                  const int idx = ((i - z0) * nj + (j - z2)) * nk + (k - z4);
+                  cd[u] = a.sfcode[(size_t)E.raw * a.zone_cap + idx];
                 }
               }
             }
@@ -649,7 +657,7 @@ __device__ __forceinline__ void map_brick(const MapArgs& a, int64_t b, MapEntry*
 #pragma unroll
         for (int u = 0; u < kMapBatch; ++u) {
           if (!(in & (1u << u))) continue;
-          const MapEntry& E = ent[perm[r0 + u]];
+          const MapEntry& E = ent[r0 + u];
           double sf;
           if (!GENERAL || (coded & (1u << u)))
             sf = map_code_sf(g.dim, cd[u]);
@@ -725,11 +733,10 @@ __device__ __forceinline__ void map_brick(const MapArgs& a, int64_t b, MapEntry*
 // The common case: bricks whose entries fit their list and all have codes.
 // Other bricks go, untouched, to a work list for k_map_general.
 __global__ __launch_bounds__(256) LBMPSM_MAP_OCC void k_map(const MapArgs a) {
-  __shared__ MapEntry ent[kBrickCap];
+  __shared__ MapEntry ent[kBrickCap];  // in list order
   __shared__ int raw[kBrickCap];
-  __shared__ int perm[kBrickCap];   // list order -> staging slot
   __shared__ int flag;
-  map_brick<false>(a, blockIdx.x, ent, raw, perm, &flag);
+  map_brick<false>(a, blockIdx.x, ent, raw, &flag);
 }
 
 // The bricks k_map passed on: overflowing lists and entries whose zones
@@ -739,10 +746,9 @@ constexpr int kMapGeneralBlocks = 256;
 __global__ __launch_bounds__(256) void k_map_general(const MapArgs a) {
   __shared__ MapEntry ent[kBrickCap];
   __shared__ int raw[kBrickCap];
-  __shared__ int perm[kBrickCap];
   const int cnt = *a.slow_count;
   for (int w = blockIdx.x; w < cnt; w += gridDim.x) {
-    map_brick<true>(a, a.slow_list[w], ent, raw, perm, nullptr);
+    map_brick<true>(a, a.slow_list[w], ent, raw, nullptr);
     __syncthreads();
   }
 }
