@@ -40,17 +40,16 @@ Q = 19
 METRIC = "MLUPS + % HBM roofline, D3Q19 PSM: FP64 vs FP32-storage at stated drag error"
 BYTES_PER_ELEM = {"f64": 8, "f32": 4, "f16": 2}
 PROF_KERNELS = ("collide_stream", "map_bin", "map_fraction", "map_particles", "hydro_reduce")
-KERNEL_NAME = {"f32": "k_collide_lowp<3,19,1,false>", "f16": "k_collide_lowp<3,19,2,false>"}
+KERNEL_NAME = {"f64": "k_collide_f64<3,19,false", "f32": "k_collide_lowp<3,19,1,false",
+               "f16": "k_collide_lowp<3,19,2,false"}
 
 
 def kernel_name(storage, F):
-    """The collide kernel the library launches: the FP64 one is formed for a
-    body force along one lattice axis (last template argument: that axis),
-    else for any force (-1)."""
-    if storage != "f64":
-        return KERNEL_NAME[storage]
+    """The collide kernel the library launches: formed for a body force along
+    one lattice axis (last template argument: that axis), else for any
+    force (-1)."""
     nz = [a for a in range(3) if F[a] != 0.0]
-    return f"k_collide_f64<3,19,false,{nz[0] if len(nz) == 1 else -1}>"
+    return f"{KERNEL_NAME[storage]},{nz[0] if len(nz) == 1 else -1}>"
 
 
 class Workload:
