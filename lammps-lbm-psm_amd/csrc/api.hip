@@ -124,16 +124,6 @@ namespace {
 #endif
 constexpr int kSfPlacement = LBMPSM_SF_PLACEMENT;
 
-// 3D map bricks (x, y, z nodes; 256 nodes, one per thread of a workgroup).
-#ifndef LBMPSM_BRICK3_I
-#define LBMPSM_BRICK3_I 4
-#endif
-#ifndef LBMPSM_BRICK3_J
-#define LBMPSM_BRICK3_J 4
-#endif
-constexpr int kBrick3[3] = {LBMPSM_BRICK3_I, LBMPSM_BRICK3_J,
-                            256 / (LBMPSM_BRICK3_I * LBMPSM_BRICK3_J)};
-static_assert(kBrick3[0] * kBrick3[1] * kBrick3[2] == 256, "256-node bricks");
 
 int fail(lbmpsm_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -850,7 +840,7 @@ int lbmpsm_create(const lbmpsm_config* cfg, lbmpsm_ctx** out) {
   if (g.dim == 3) {
     B.bi = kBrick3[0]; B.bj = kBrick3[1]; B.bk = kBrick3[2];
   } else {
-    B.bi = 16; B.bj = 16; B.bk = 1;
+    B.bi = kBrick2[0]; B.bj = kBrick2[1]; B.bk = kBrick2[2];
   }
   B.nbi = (g.sx + B.bi - 1) / B.bi;
   B.nbj = (g.sy + B.bj - 1) / B.bj;

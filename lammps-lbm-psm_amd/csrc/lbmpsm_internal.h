@@ -122,6 +122,18 @@ struct ParticleDev {
 // zone touches it (unordered, at most kBrickCap; longer lists make the map
 // kernel scan the whole particle list for that brick).
 constexpr int kBrickCap = 64;
+// Brick edge lengths (x, y, z nodes; 256 nodes, one per thread of a map
+// workgroup): 3D, and 2D (z extent 1).
+#ifndef LBMPSM_BRICK3_I
+#define LBMPSM_BRICK3_I 4
+#endif
+#ifndef LBMPSM_BRICK3_J
+#define LBMPSM_BRICK3_J 4
+#endif
+constexpr int kBrick3[3] = {LBMPSM_BRICK3_I, LBMPSM_BRICK3_J,
+                            256 / (LBMPSM_BRICK3_I * LBMPSM_BRICK3_J)};
+constexpr int kBrick2[3] = {16, 16, 1};
+static_assert(kBrick3[0] * kBrick3[1] * kBrick3[2] == 256, "256-node bricks");
 struct BrickDev {
   int bi = 0, bj = 0, bk = 0;  // brick edge lengths (nodes)
   int nbi = 0, nbj = 0, nbk = 0;

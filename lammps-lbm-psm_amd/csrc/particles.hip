@@ -416,8 +416,16 @@ struct SfLineShared {
 
 __global__ __launch_bounds__(256) void k_entry_sf_lines(const MapArgs a, int planes) {
   __shared__ SfLineShared sh;
-  const int e = (int)blockIdx.x / planes;
-  const int li = (int)blockIdx.x - e * planes;
+  // grid (planes, entries), or one dimension of entries x planes when the
+  // entries exceed the grid's second dimension
+  int e, li;
+  if (gridDim.y > 1) {
+    li = (int)blockIdx.x;
+    e = (int)blockIdx.y;
+  } else {
+    e = (int)blockIdx.x / planes;
+    li = (int)blockIdx.x - e * planes;
+  }
   if (e >= a.n) return;
   const Geom& g = a.g;
   const int* z = a.zone + 6 * e;
@@ -557,9 +565,14 @@ constexpr int kMapBatch = LBMPSM_MAP_BATCH;
 #define LBMPSM_MAP_OCC
 #endif
 
-template <bool GENERAL>
-__device__ __forceinline__ void map_brick(const MapArgs& a, int64_t b, MapEntry* ent, int* raw,
-                                          int* flag) {
+template <bool GENERAL, int DIM>
+__device__ __forceinline__ void map_brick(const MapArgs& a, int64_t b, int bx, int by, int bz,
+                                          MapEntry* ent, int* raw, int* flag) {
+  // the brick's shape is fixed per dimension, so a thread's node within it
+  // comes from shifts and masks
+  constexpr int BI = DIM == 3 ? kBrick3[0] : kBrick2[0];
+  constexpr int BJ = DIM == 3 ? kBrick3[1] : kBrick2[1];
+  constexpr int BK = DIM == 3 ? kBrick3[2] : kBrick2[2];
   const int tid = threadIdx.x;
   // independent loads first: the brick's count and list, the node's slots
   const int m = a.count[b];
@@ -573,14 +586,10 @@ __device__ __forceinline__ void map_brick(const MapArgs& a, int64_t b, MapEntry*
   const int64_t plane = g.plane;
   const bool listed = m <= kBrickCap;
 
-  const int bb = (int)b;
-  const int bx = bb / (a.nbj * a.nbk);
-  const int by = (bb / a.nbk) % a.nbj;
-  const int bz = bb % a.nbk;
-  const int lk = tid % a.bk;
-  const int lj = (tid / a.bk) % a.bj;
-  const int li = tid / (a.bk * a.bj);
-  const int i = bx * a.bi + li, j = by * a.bj + lj, k = bz * a.bk + lk;
+  const int lk = tid % BK;
+  const int lj = (tid / BK) % BJ;
+  const int li = tid / (BK * BJ);
+  const int i = bx * BI + li, j = by * BJ + lj, k = bz * BK + lk;
   const bool valid = i < g.sx && j < g.sy && k < g.sz;
   const int n = valid ? (i * g.sy + j) * g.sz + k : 0;
   NodeSlots S;
@@ -732,23 +741,43 @@ __device__ __forceinline__ void map_brick(const MapArgs& a, int64_t b, MapEntry*
 
 // The common case: bricks whose entries fit their list and all have codes.
 // Other bricks go, untouched, to a work list for k_map_general.
+// GRID3: the grid is (z, y, x) bricks; else one dimension of bricks.
+template <int DIM, bool GRID3>
 __global__ __launch_bounds__(256) LBMPSM_MAP_OCC void k_map(const MapArgs a) {
   __shared__ MapEntry ent[kBrickCap];  // in list order
   __shared__ int raw[kBrickCap];
   __shared__ int flag;
-  map_brick<false>(a, blockIdx.x, ent, raw, &flag);
+  int bx, by, bz;
+  int64_t b;
+  if constexpr (GRID3) {
+    bz = blockIdx.x;
+    by = blockIdx.y;
+    bx = blockIdx.z;
+    b = ((int64_t)bx * a.nbj + by) * a.nbk + bz;
+  } else {
+    b = blockIdx.x;
+    bx = (int)(b / ((int64_t)a.nbj * a.nbk));
+    by = (int)((b / a.nbk) % a.nbj);
+    bz = (int)(b % a.nbk);
+  }
+  map_brick<false, DIM>(a, b, bx, by, bz, ent, raw, &flag);
 }
 
 // The bricks k_map passed on: overflowing lists and entries whose zones
 // have no codes (solid fractions computed here, cell by cell).  A fixed grid
 // walks the work list; an empty list costs one load per workgroup.
 constexpr int kMapGeneralBlocks = 256;
+template <int DIM>
 __global__ __launch_bounds__(256) void k_map_general(const MapArgs a) {
   __shared__ MapEntry ent[kBrickCap];
   __shared__ int raw[kBrickCap];
   const int cnt = *a.slow_count;
   for (int w = blockIdx.x; w < cnt; w += gridDim.x) {
-    map_brick<true>(a, a.slow_list[w], ent, raw, nullptr);
+    const int64_t b = a.slow_list[w];
+    const int bx = (int)(b / ((int64_t)a.nbj * a.nbk));
+    const int by = (int)((b / a.nbk) % a.nbj);
+    const int bz = (int)(b % a.nbk);
+    map_brick<true, DIM>(a, b, bx, by, bz, ent, raw, nullptr);
     __syncthreads();
   }
 }
@@ -955,8 +984,9 @@ hipError_t launch_entry_sf(const MapArgs& a, hipStream_t s) {
   if (a.g.dim == 3) {
     // every coded zone is at most zone_span planes wide in x
     const int planes = std::max(1, std::min(a.zone_span, a.g.sx));
-    hipLaunchKernelGGL(k_entry_sf_lines, dim3((unsigned)((int64_t)a.n * planes)), dim3(256), 0,
-                       s, a, planes);
+    const dim3 grid = a.n > 1 && a.n <= 65535 ? dim3((unsigned)planes, (unsigned)a.n)
+                                              : dim3((unsigned)((int64_t)a.n * planes));
+    hipLaunchKernelGGL(k_entry_sf_lines, grid, dim3(256), 0, s, a, planes);
   }
   if (a.g.dim == 2 || a.zone_span > kLineMax) {  // zones the line kernel leaves
     const int64_t chunks = (a.zone_cap + kSfChunk - 1) / kSfChunk;
@@ -969,9 +999,22 @@ hipError_t launch_map(const MapArgs& a, int64_t nbricks, hipStream_t s) {
   // without entries no node changes state (and k_prep_bin, which zeroes the
   // general kernel's work list, did not run)
   if (nbricks <= 0 || a.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_map, dim3((unsigned)nbricks), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(k_map_general, dim3((unsigned)std::min<int64_t>(nbricks, kMapGeneralBlocks)),
-                     dim3(256), 0, s, a);
+  // the brick shapes the kernels are compiled for
+  const bool d3 = a.g.dim == 3;
+  const int* br = d3 ? kBrick3 : kBrick2;
+  if (a.bi != br[0] || a.bj != br[1] || a.bk != br[2]) return hipErrorInvalidValue;
+  const bool grid3 = a.nbj <= 65535 && a.nbi <= 65535;
+  const dim3 g3((unsigned)a.nbk, (unsigned)a.nbj, (unsigned)a.nbi), g1((unsigned)nbricks);
+  const dim3 gg((unsigned)std::min<int64_t>(nbricks, kMapGeneralBlocks));
+  if (d3) {
+    if (grid3) hipLaunchKernelGGL((k_map<3, true>), g3, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_map<3, false>), g1, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_map_general<3>, gg, dim3(256), 0, s, a);
+  } else {
+    if (grid3) hipLaunchKernelGGL((k_map<2, true>), g3, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_map<2, false>), g1, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_map_general<2>, gg, dim3(256), 0, s, a);
+  }
   return hipGetLastError();
 }
 
