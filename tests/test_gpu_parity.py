@@ -89,6 +89,29 @@ def test_fluid_periodic_guo_2d():
     assert np.array_equal(lat.populations(), ora.populations())
 
 
+@pytest.mark.parametrize("dim,F", [(3, (0.0, 2e-6, 0.0)), (3, (0.0, 0.0, -3e-6)),
+                                   (2, (0.0, 2e-6, 0.0))])
+def test_body_force_along_one_axis(dim, F):
+    """The collides formed for a body force along a single lattice axis (the
+    y and z kernels; x runs in most other cases) with moving spheres, two of
+    them sharing rim nodes: populations, slots and F/T/S against the oracle,
+    and fluid-only steps bit for bit."""
+    n = (24, 22, 20) if dim == 3 else (40, 36, 1)
+    per = (1, 1, 1)
+    box = lattice_box(dim, n, per)
+    if dim == 3:
+        pos = np.array([[8.0, 10.2, 9.9], [15.6, 10.4, 10.1], [18.5, 3.3, 15.2]])
+        radii = [3.7, 3.6, 3.2]
+    else:
+        pos = np.array([[12.0, 14.2, 0.0], [21.6, 14.4, 0.0], [33.5, 5.3, 0.0]])
+        radii = [4.7, 4.6, 4.2]
+    atoms = random_atoms(box, radii, seed=17, positions=pos, vmax=1e-2, wmax=1e-3)
+    lat, ora, out = run_pair(dim, n, per, 0.7, F, "none", atoms, 8, label=f"axis force {F}")
+    compare(lat, ora, out)
+    lat2, ora2, _ = run_pair(dim, n, per, 0.7, F, "none", None, 6, label=f"axis force fluid {F}")
+    assert np.array_equal(lat2.populations(), ora2.populations())
+
+
 def test_spheres_shear_3d_moving():
     n = (28, 26, 24)
     per = (1, 0, 1)

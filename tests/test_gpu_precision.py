@@ -104,6 +104,37 @@ def test_reduced_storage_with_shared_nodes(storage, tol):
     assert eF <= tol and eT <= tol
 
 
+def run_axis_force(storage, F, steps=200):
+    n = (32, 32, 32)
+    per = (1, 1, 1)
+    box = lattice_box(3, n, per)
+    atoms = random_atoms(box, [5.0, 4.5, 5.5], seed=23, vmax=1e-3, wmax=1e-4, gap=2.0)
+    lat = lbmpsm.Lattice(lbmpsm.lattice_units_config(3, n, per, 0.7, storage=storage, F_lbm=F))
+    for s in range(steps):
+        H.move_linear(atoms, box, s, 1.0)
+        lst = H.build_list(atoms, box, 9.0)
+        lat.set_particles(lst.tag, lst.x, lst.v, lst.omega, lst.radius)
+        lat.step()
+    Fh, T, _ = lat.hydro()
+    lat.close()
+    return Fh, T
+
+
+@pytest.mark.parametrize("F", [(0.0, 3e-6, 0.0), (0.0, 0.0, 3e-6)])
+def test_reduced_storage_axis_force(F):
+    """The FP32 and FP16 collides formed for a body force along y or z (the
+    x form runs in the other cases) against FP64: drag and torque of 3 moving
+    spheres in a periodic 32^3 box after 200 steps, bars as above."""
+    print(f"\n[case] precision, body force {F}", flush=True)
+    F0, T0 = run_axis_force("f64", F)
+    for storage, tol in (("f32", 1e-5), ("f16", 2e-3)):
+        Fh, T = run_axis_force(storage, F)
+        eF = np.max(np.linalg.norm(Fh - F0, axis=1)) / np.max(np.linalg.norm(F0, axis=1))
+        eT = np.max(np.linalg.norm(T - T0, axis=1)) / np.max(np.linalg.norm(T0, axis=1))
+        print(f"  {storage}: drag {eF:.2e} torque {eT:.2e}", flush=True)
+        assert eF <= tol and eT <= tol
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_fp32_mass_kept_at_raised_density(dim):
     """FP32 storage keeps the mass of a periodic box whose density sits 0.1
