@@ -39,7 +39,9 @@ struct Geom {
 #endif
 template <int Q>
 __host__ __device__ inline size_t pop_index_f64(int q, int64_t plane, unsigned n) {
-  if constexpr (LBMPSM_F64_BLOCKED) return ((size_t)(n >> 6) * Q + (size_t)q) * 64 + (n & 63u);
+  // the node's part first, so that the directions of one node share it
+  if constexpr (LBMPSM_F64_BLOCKED)
+    return ((size_t)(n >> 6) * (Q * 64) + (n & 63u)) + (size_t)q * 64;
   else return (size_t)q * (size_t)plane + n;
 }
 
