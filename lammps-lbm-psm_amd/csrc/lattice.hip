@@ -43,7 +43,7 @@
 #define LBMPSM_F64_SPLIT 1  // 1: FP64 particle waves without a second particle skip its terms
 #endif
 #ifndef LBMPSM_LOWP_WAVES
-#define LBMPSM_LOWP_WAVES 5  // > 0: waves per SIMD asked of the bulk k_collide_lowp
+#define LBMPSM_LOWP_WAVES 4  // > 0: waves per SIMD asked of the bulk k_collide_lowp
 #endif
 #if LBMPSM_LOWP_WAVES > 0
 // the once-per-run post-stream variant is left to the compiler (it would spill)
@@ -237,7 +237,7 @@ template <int ST, int Q>
 __device__ __forceinline__ size_t pop_index(int q, int64_t plane, unsigned n) {
   if constexpr (ST == ST_F64) return pop_index_f64<Q>(q, plane, n);
   else if constexpr (!LBMPSM_LOWP_BLOCKED) return (size_t)q * (size_t)plane + n;
-  else return ((size_t)(n >> 6) * Q + (size_t)q) * 64 + (n & 63u);
+  else return ((size_t)(n >> 6) * (Q * 64) + (n & 63u)) + (size_t)q * 64;
 }
 template <int ST>
 constexpr bool kPlanar = ST == ST_F64 ? !LBMPSM_F64_BLOCKED : !LBMPSM_LOWP_BLOCKED;
