@@ -262,17 +262,64 @@ __device__ __forceinline__ typename StoreT<ST>::T* pop_ptr_w(void* base, int q, 
     return (T*)base + pop_index<ST, Q>(q, plane, rowb + col);
 }
 
+// Rows whose length is a multiple of 64 (ALN) start on a block boundary, so
+// the blocked index of (row start rowb, column col) splits into a uniform
+// part, the row's first block plus the direction's run, and a lane part
+// that depends on the column only: one 32-bit lane offset then serves
+// every direction pulled from the same column, and each access is a scalar
+// base plus that offset.
+template <int ST, int Q>
+__device__ __forceinline__ unsigned col_off(unsigned col) {
+  return ((col >> 6) * (unsigned)(Q * 64) + (col & 63u)) * (unsigned)sizeof(typename StoreT<ST>::T);
+}
+// The uniform part as a scalar 64-bit base: passing it through readfirstlane
+// keeps the compiler from folding the direction's run into the lane offset
+// (a 64-bit vector add per access when the run lies beyond the 4 KiB
+// immediate offset of a global access).
+// The result is marked as global memory (address space 1), which an integer
+// round trip would otherwise lose (flat accesses).
+__device__ __forceinline__ char* scalar_base(const void* base, size_t u) {
+  using gchar = __attribute__((address_space(1))) char;
+  const uint64_t b = (uint64_t)base + u;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return (char*)(gchar*)(((uint64_t)hi << 32) | lo);
+}
+// Direction q's run starts q * 64 elements into the block; the part of that
+// below 4 KiB is left to the access's immediate offset, so the directions
+// of one row share a few scalar bases.
+template <int ST, int q>
+constexpr int kRunHi = (q * 64 * (int)sizeof(typename StoreT<ST>::T)) & ~4095;
+template <int ST, int q>
+constexpr int kRunLo = (q * 64 * (int)sizeof(typename StoreT<ST>::T)) & 4095;
+template <int ST, int Q, int q>
+__device__ __forceinline__ const typename StoreT<ST>::T* pop_ptr_al(const void* base,
+                                                                    unsigned rowb, unsigned off) {
+  using T = typename StoreT<ST>::T;
+  const size_t u = (size_t)(rowb >> 6) * (Q * 64 * sizeof(T)) + kRunHi<ST, q>;
+  return (const T*)(scalar_base(base, u) + off + kRunLo<ST, q>);
+}
+template <int ST, int Q, int q>
+__device__ __forceinline__ typename StoreT<ST>::T* pop_ptr_al_w(void* base, unsigned rowb,
+                                                               unsigned off) {
+  using T = typename StoreT<ST>::T;
+  const size_t u = (size_t)(rowb >> 6) * (Q * 64 * sizeof(T)) + kRunHi<ST, q>;
+  return (T*)(scalar_base(base, u) + off + kRunLo<ST, q>);
+}
+
 // Pull the post-stream populations of the node in column c of row r, then
 // apply the Zou-He faces.  In post-stream mode the node's own values are
 // read and no face is applied (initial state / state set by the host).
 // PS: 0 pull, 1 post-stream, 2 as a.poststream says (the bulk kernels are
 // compiled for one mode, which keeps the other path's loads out of their
-// wait-count bookkeeping).
-template <int DIM, int Q, int ST, bool FACES, int PS = 2>
+// wait-count bookkeeping).  ALN: rows are a whole number of blocks long
+// (col_off).
+template <int DIM, int Q, int ST, bool FACES, int PS = 2, bool ALN = false>
 __device__ __forceinline__ void gather_row(const CollideArgs& a, const Row& r, int c,
                                            typename StoreT<ST>::C* f) {
   using T = typename StoreT<ST>::T;
   using C = typename StoreT<ST>::C;
+  constexpr bool AL = ALN && !kPlanar<ST>;
   const Geom& g = a.g;
   const int64_t plane = g.plane;
   if (PS == 1 || (PS == 2 && a.poststream)) {
@@ -286,6 +333,12 @@ __device__ __forceinline__ void gather_row(const CollideArgs& a, const Row& r, i
   int cm, cp;
   bool vcm, vcp;
   nbr(c, r.ncols, per_c, cm, vcm, cp, vcp);
+  unsigned oc = 0, om = 0, op = 0;
+  if constexpr (AL) {
+    oc = col_off<ST, Q>((unsigned)c);
+    om = col_off<ST, Q>((unsigned)cm);
+    op = col_off<ST, Q>((unsigned)cp);
+  }
   Unroll<0, Q>::run([&]<int q>() {
     constexpr int ex = VelSet<Q>::ex[q], ey = VelSet<Q>::ey[q], ez = VelSet<Q>::ez[q];
     // source node = node - e_q; the column axis is z in 3D and y in 2D
@@ -295,7 +348,10 @@ __device__ __forceinline__ void gather_row(const CollideArgs& a, const Row& r, i
     const unsigned col = (unsigned)(ec == 1 ? cm : (ec == -1 ? cp : c));
     const bool cok = ec == 1 ? vcm : (ec == -1 ? vcp : true);
     const bool rok = (r.okmask >> (3 * (1 - ea) + (1 - eb))) & 1u;
-    const T* src = pop_ptr<ST, Q>(a.fin, q, plane, row_base<DIM>(g, r, 1 - ea, 1 - eb), col);
+    const unsigned rs = row_base<DIM>(g, r, 1 - ea, 1 - eb);
+    const T* src;
+    if constexpr (AL) src = pop_ptr_al<ST, Q, q>(a.fin, rs, ec == 1 ? om : (ec == -1 ? op : oc));
+    else src = pop_ptr<ST, Q>(a.fin, q, plane, rs, col);
     // both values unconditionally (src stays inside the buffer: nbr keeps
     // the coordinate off a non-periodic edge), then a select: with no branch
     // around the load, the loads of all directions are in flight together
@@ -395,7 +451,7 @@ __device__ __forceinline__ void load_slots_f64(const CollideArgs& a, unsigned n,
 }
 
 // Collision of one node whose post-stream populations are in f[] (FP64).
-template <int DIM, int Q, int FAX>
+template <int DIM, int Q, int FAX, bool ALN>
 __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, unsigned rb,
                                             unsigned uc, SlotsF64& S);
 
@@ -428,7 +484,7 @@ __device__ __forceinline__ double guo_f64(double ux, double uy, double uz, doubl
 // Zou-He faces has them applied, in table order, to its pulled populations;
 // its pulls are part of the row's coalesced loads (a z face node would
 // otherwise be a strided access of its own).
-template <int DIM, int Q, bool POST, int FAX>
+template <int DIM, int Q, bool POST, int FAX, bool ALN>
 __global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_f64(const CollideArgs a) {
   const Geom& g = a.g;
   Row r;
@@ -442,19 +498,25 @@ __global__ __launch_bounds__(256) LBMPSM_OCC void k_collide_f64(const CollideArg
   // them: a wave with particle nodes has both in flight with the populations
   const uint8_t m = a.mask[rb + (unsigned)c];
   double f[Q];
-  gather_row<DIM, Q, ST_F64, false, POST ? 1 : 0>(a, r, c, f);
+  gather_row<DIM, Q, ST_F64, false, POST ? 1 : 0, ALN>(a, r, c, f);
   SlotsF64 S;
   load_slots_f64(a, rb + (unsigned)c, m, S);
   if (__any(face)) apply_faces<Q, double>(f, a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c,
                                           DIM == 3 ? c : 0, 0.0);
-  collide_f64<DIM, Q, FAX>(a, f, rb, (unsigned)c, S);
+  collide_f64<DIM, Q, FAX, ALN>(a, f, rb, (unsigned)c, S);
 }
 
-template <int DIM, int Q, int FAX>
+template <int DIM, int Q, int FAX, bool ALN>
 __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, unsigned rb,
                                             unsigned uc, SlotsF64& S) {
   const Geom& g = a.g;
   const int64_t plane = g.plane;
+  constexpr bool AL = ALN && !kPlanar<ST_F64>;
+  const unsigned oc = AL ? col_off<ST_F64, Q>(uc) : 0u;
+  auto dst = [&]<int q>() __attribute__((always_inline)) {
+    if constexpr (AL) return pop_ptr_al_w<ST_F64, Q, q>(a.fout, rb, oc);
+    else return pop_ptr_w<ST_F64, Q>(a.fout, q, plane, rb, uc);
+  };
   const double omega = a.omega, inv2 = a.invCs2, inv4 = a.invCs4;
   // The reference's quadratic term (e.u)(e.u) 0.5 invCs4 (fluid) or
   // 0.5 (e.u)(e.u) invCs4 (solid) halves a product exactly, so it equals
@@ -498,7 +560,7 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
         double out = f[q] + (feq - f[q]) * omega;
         if (FAX >= 0 || a.guo)
           out = out + a.guo_pref * guo_f64<Q, q, FAX>(ux, uy, uz, eu, F0, F1, F2, inv2, inv4);
-        st<ST_F64>(pop_ptr_w<ST_F64, Q>(a.fout, q, plane, rb, uc), out, 1.0f);
+        st<ST_F64>(dst.template operator()<q>(), out, 1.0f);
       });
     } else {
       const double u1sq = 0.5 * (us1[0] * us1[0] + us1[1] * us1[1] + us1[2] * us1[2]) * inv2;
@@ -544,7 +606,7 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
           if constexpr (TWO) out = out + B2 * sc2;
           if (FAX >= 0 || a.guo)
             out = out + a.guo_pref * guo_f64<Q, q, FAX>(ux, uy, uz, eu, F0, F1, F2, inv2, inv4);
-          st<ST_F64>(pop_ptr_w<ST_F64, Q>(a.fout, q, plane, rb, uc), out, 1.0f);
+          st<ST_F64>(dst.template operator()<q>(), out, 1.0f);
           // Node force on each slot, src/LBM_PSM_BGK_dynamics.h:101-113.
           if (o1) {
             const double t = -B1 * sc1;
@@ -625,7 +687,7 @@ __device__ __forceinline__ void load_slots_f32(const CollideArgs& a, unsigned n,
   }
 }
 
-template <int DIM, int Q, int ST, int FAX>
+template <int DIM, int Q, int ST, int FAX, bool ALN>
 __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, unsigned rb,
                                              unsigned uc, const SlotsF32& S);
 
@@ -744,7 +806,7 @@ template <int DIM, int Q, bool EXACT, int FAX, typename Emit>
 __device__ __forceinline__ void collide_lowp_node(const CollideArgs& a, float* f, unsigned n,
                                                   const SlotsF32& S, Emit&& emit);
 
-template <int DIM, int Q, int ST, bool POST, int FAX>
+template <int DIM, int Q, int ST, bool POST, int FAX, bool ALN>
 __global__ __launch_bounds__(256) LBMPSM_LOWP_OCC void k_collide_lowp(const CollideArgs a) {
   const Geom& g = a.g;
   Row r;
@@ -757,20 +819,24 @@ __global__ __launch_bounds__(256) LBMPSM_LOWP_OCC void k_collide_lowp(const Coll
   // slot mask before the pulls, slot data right after them (as in FP64)
   const uint8_t m = a.mask[rb + (unsigned)c];
   float f[Q];
-  gather_row<DIM, Q, ST, false, POST ? 1 : 0>(a, r, c, f);
+  gather_row<DIM, Q, ST, false, POST ? 1 : 0, ALN>(a, r, c, f);
   SlotsF32 S;
   load_slots_f32(a, rb + (unsigned)c, m, S);
   if (__any(face)) apply_faces<Q, float>(f, a.faces, a.nfaces, r.i, DIM == 3 ? r.j : c,
                                          DIM == 3 ? c : 0, a.lowp_c[1]);
-  collide_lowp<DIM, Q, ST, FAX>(a, f, rb, (unsigned)c, S);
+  collide_lowp<DIM, Q, ST, FAX, ALN>(a, f, rb, (unsigned)c, S);
 }
 
-template <int DIM, int Q, int ST, int FAX>
+template <int DIM, int Q, int ST, int FAX, bool ALN>
 __device__ __forceinline__ void collide_lowp(const CollideArgs& a, float* f, unsigned rb,
                                              unsigned uc, const SlotsF32& S) {
   const int64_t plane = a.g.plane;
+  constexpr bool AL = ALN && !kPlanar<ST>;
+  const unsigned oc = AL ? col_off<ST, Q>(uc) : 0u;
   collide_lowp_node<DIM, Q, ST == ST_F32, FAX>(a, f, rb + uc, S, [&](auto qc, float out) {
-    st<ST>(pop_ptr_w<ST, Q>(a.fout, decltype(qc)::value, plane, rb, uc), out, a.scale);
+    constexpr int q = decltype(qc)::value;
+    if constexpr (AL) st<ST>(pop_ptr_al_w<ST, Q, q>(a.fout, rb, oc), out, a.scale);
+    else st<ST>(pop_ptr_w<ST, Q>(a.fout, q, plane, rb, uc), out, a.scale);
   });
 }
 
@@ -1082,6 +1148,32 @@ static inline int64_t rows_of(const Geom& g) {
   return g.dim == 3 ? (int64_t)g.sx * g.sy : (int64_t)g.sx;
 }
 
+// One bulk kernel of the storage form ST (FP64 or reduced).
+template <int DIM, int Q, int ST, bool POST, int FAX, bool ALN>
+static void launch_bulk(dim3 grid, dim3 block, hipStream_t s, const CollideArgs& a) {
+  if constexpr (ST == ST_F64)
+    hipLaunchKernelGGL((k_collide_f64<DIM, Q, POST, FAX, ALN>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST, POST, FAX, ALN>), grid, block, 0, s, a);
+}
+
+template <int DIM, int Q, int ST, bool ALN>
+static void launch_fax(int fax, dim3 grid, dim3 block, hipStream_t s, const CollideArgs& a) {
+  if (fax == 0) launch_bulk<DIM, Q, ST, false, 0, ALN>(grid, block, s, a);
+  else if (fax == 1) launch_bulk<DIM, Q, ST, false, 1, ALN>(grid, block, s, a);
+  else if (DIM == 3 && fax == 2)
+    launch_bulk<DIM, Q, ST, false, DIM == 3 ? 2 : -1, ALN>(grid, block, s, a);
+  else launch_bulk<DIM, Q, ST, false, -1, ALN>(grid, block, s, a);
+}
+
+template <int DIM, int Q, int ST>
+static void launch_storage(int fax, bool aln, dim3 grid, dim3 block, hipStream_t s,
+                           const CollideArgs& a) {
+  if (a.poststream) launch_bulk<DIM, Q, ST, true, -1, false>(grid, block, s, a);
+  else if (aln) launch_fax<DIM, Q, ST, true>(fax, grid, block, s, a);
+  else launch_fax<DIM, Q, ST, false>(fax, grid, block, s, a);
+}
+
 template <int DIM, int Q>
 static hipError_t collide_dim(int storage, const CollideArgs& a, hipStream_t s) {
   const int64_t rows = rows_of(a.g);
@@ -1096,32 +1188,13 @@ static hipError_t collide_dim(int storage, const CollideArgs& a, hipStream_t s) 
     if (nz == 1) fax = a.F[0] != 0.0 ? 0 : (a.F[1] != 0.0 ? 1 : 2);
     if (DIM == 2 && fax == 2) fax = -1;
   }
+  // rows a whole number of 64-node blocks long take the split addressing
+  // (its lane offsets, below ncols * Q * 8 bytes, must fit 32 bits)
+  const bool aln = ncols % 64 == 0 && (int64_t)ncols * Q * 8 < ((int64_t)1 << 32);
   switch (storage) {
-    case ST_F64: {
-      if (a.poststream) hipLaunchKernelGGL((k_collide_f64<DIM, Q, true, -1>), grid, block, 0, s, a);
-      else if (fax == 0) hipLaunchKernelGGL((k_collide_f64<DIM, Q, false, 0>), grid, block, 0, s, a);
-      else if (fax == 1) hipLaunchKernelGGL((k_collide_f64<DIM, Q, false, 1>), grid, block, 0, s, a);
-      else if (DIM == 3 && fax == 2)
-        hipLaunchKernelGGL((k_collide_f64<DIM, Q, false, DIM == 3 ? 2 : -1>), grid, block, 0, s, a);
-      else hipLaunchKernelGGL((k_collide_f64<DIM, Q, false, -1>), grid, block, 0, s, a);
-      break;
-    }
-    case ST_F32:
-      if (a.poststream) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32, true, -1>), grid, block, 0, s, a);
-      else if (fax == 0) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32, false, 0>), grid, block, 0, s, a);
-      else if (fax == 1) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32, false, 1>), grid, block, 0, s, a);
-      else if (DIM == 3 && fax == 2)
-        hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32, false, DIM == 3 ? 2 : -1>), grid, block, 0, s, a);
-      else hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F32, false, -1>), grid, block, 0, s, a);
-      break;
-    case ST_F16:
-      if (a.poststream) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16, true, -1>), grid, block, 0, s, a);
-      else if (fax == 0) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16, false, 0>), grid, block, 0, s, a);
-      else if (fax == 1) hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16, false, 1>), grid, block, 0, s, a);
-      else if (DIM == 3 && fax == 2)
-        hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16, false, DIM == 3 ? 2 : -1>), grid, block, 0, s, a);
-      else hipLaunchKernelGGL((k_collide_lowp<DIM, Q, ST_F16, false, -1>), grid, block, 0, s, a);
-      break;
+    case ST_F64: launch_storage<DIM, Q, ST_F64>(fax, aln, grid, block, s, a); break;
+    case ST_F32: launch_storage<DIM, Q, ST_F32>(fax, aln, grid, block, s, a); break;
+    case ST_F16: launch_storage<DIM, Q, ST_F16>(fax, aln, grid, block, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -112,6 +112,36 @@ def test_body_force_along_one_axis(dim, F):
     assert np.array_equal(lat2.populations(), ora2.populations())
 
 
+@pytest.mark.parametrize("dim,n,per,F,bc,pos,radii", [
+    (3, (14, 13, 128), (1, 1, 1), (1e-5, -2e-6, 3e-6), "none",
+     [[7.0, 6.4, 63.7], [6.6, 6.8, 2.1], [7.2, 6.0, 100.3]], [3.2, 2.8, 3.0]),
+    (3, (13, 13, 62), (1, 1, 0), (0.0, 2e-6, 0.0), "none",
+     [[6.5, 6.2, 31.5], [6.0, 5.9, 8.0], [6.8, 6.4, 52.2]], [3.3, 3.0, 2.9]),
+    (3, (14, 14, 64), (1, 0, 1), (0.0, 0.0, 0.0), "shear",
+     [[7.0, 7.1, 63.2], [6.8, 6.9, 30.0]], [3.5, 3.2]),
+    (2, (20, 128, 1), (1, 1, 1), (1e-5, 2e-6, 0.0), "none",
+     [[10.0, 63.6, 0.0], [9.4, 12.0, 0.0], [10.5, 120.4, 0.0]], [4.5, 4.0, 4.2]),
+    (2, (24, 62, 1), (1, 0, 1), (0.0, 0.0, 0.0), "shear",
+     [[12.0, 31.0, 0.0], [8.0, 10.3, 0.0]], [5.0, 4.5]),
+])
+def test_rows_of_whole_blocks(dim, n, per, F, bc, pos, radii):
+    """Stored rows (z in 3D, y in 2D) a whole number of 64-node blocks long
+    (periodic 64 or 128 nodes, or 62 interior nodes plus the two wall
+    layers) take the collide's split addressing, a per-row scalar base plus
+    a per-column lane offset.  Periodic and walled row ends, rows of two
+    blocks whose pulls cross the block boundary, spheres on that boundary
+    and on the periodic seam, general and single-axis body forces: against
+    the oracle.  Each periodic extent exceeds 2r + 5 nodes, so the map
+    zones of a sphere's own images stay apart (with them overlapping, the
+    reference's halo nodes can end a map with slots that differ from their
+    periodic counterparts, which this library does not store)."""
+    box = lattice_box(dim, n, per)
+    atoms = random_atoms(box, radii, seed=29, positions=np.array(pos, float), vmax=1e-2,
+                         wmax=1e-3)
+    lat, ora, out = run_pair(dim, n, per, 0.7, F, bc, atoms, 8, label=f"whole blocks {n}")
+    compare(lat, ora, out)
+
+
 def test_spheres_shear_3d_moving():
     n = (28, 26, 24)
     per = (1, 0, 1)
