@@ -54,9 +54,20 @@
 #ifndef LBMPSM_ROW_WAVES
 #define LBMPSM_ROW_WAVES 1  // waves per bulk workgroup; 0: the chunks of one row, at most 4
 #endif
+#ifndef LBMPSM_ROW_WAVES_LOWP
+#define LBMPSM_ROW_WAVES_LOWP 2  // the same for the FP32 / FP16 kernels on a large lattice
+#endif
+#ifndef LBMPSM_LARGE_WAVES
+#define LBMPSM_LARGE_WAVES 262144  // a large lattice: at least this many 64-node chunks
+#endif
 static_assert(LBMPSM_ROW_WAVES >= 0 && LBMPSM_ROW_WAVES <= 4, "bulk workgroups hold at most 4 waves");
+static_assert(LBMPSM_ROW_WAVES_LOWP >= 0 && LBMPSM_ROW_WAVES_LOWP <= 4,
+              "bulk workgroups hold at most 4 waves");
 #ifndef LBMPSM_XCD_SWIZZLE
-#define LBMPSM_XCD_SWIZZLE 0  // 1: give each XCD a contiguous range of workgroups
+#define LBMPSM_XCD_SWIZZLE 1  // 1: give each XCD a contiguous range of workgroups
+#endif
+#ifndef LBMPSM_XCD_SWIZZLE_MIN
+#define LBMPSM_XCD_SWIZZLE_MIN 131072  // ... in grids of at least this many workgroups
 #endif
 
 namespace lbmpsm {
@@ -395,9 +406,14 @@ __device__ __forceinline__ int bulk_wave() {
   int b = (int)blockIdx.x;
 #if LBMPSM_XCD_SWIZZLE
   // workgroups are dealt round-robin over the 8 XCDs; renumber them so
-  // that each XCD (and its L2) walks one contiguous range of rows
-  const int nb = (int)gridDim.x, per = nb >> 3, rem = nb & 7, x = b & 7;
-  b = x * per + min(x, rem) + (b >> 3);
+  // that each XCD (and its L2) walks one contiguous range of rows, whose
+  // neighbouring rows it then reads from its own L2 (a large lattice; a
+  // small one is served from the shared last-level cache anyway)
+  const int nb = (int)gridDim.x;
+  if (nb >= LBMPSM_XCD_SWIZZLE_MIN) {
+    const int per = nb >> 3, rem = nb & 7, x = b & 7;
+    b = x * per + min(x, rem) + (b >> 3);
+  }
 #endif
   // wave-uniform by construction; readfirstlane lets the compiler keep the
   // row set-up in scalar registers and address with scalar bases
@@ -1179,7 +1195,12 @@ static hipError_t collide_dim(int storage, const CollideArgs& a, hipStream_t s) 
   const int64_t rows = rows_of(a.g);
   const int ncols = DIM == 3 ? a.g.sz : a.g.sy;
   const int64_t waves = rows * ((ncols + 63) / 64);
-  const int wpb = LBMPSM_ROW_WAVES > 0 ? LBMPSM_ROW_WAVES : std::min((ncols + 63) / 64, 4);
+  // measured: two-wave workgroups help the reduced forms on a large lattice
+  // (where the particle kernels beside the collide take a larger share) and
+  // cost them a few percent on a small one
+  const int rw = storage != ST_F64 && waves >= LBMPSM_LARGE_WAVES ? LBMPSM_ROW_WAVES_LOWP
+                                                                  : LBMPSM_ROW_WAVES;
+  const int wpb = rw > 0 ? rw : std::min((ncols + 63) / 64, 4);
   dim3 grid((unsigned)((waves + wpb - 1) / wpb)), block((unsigned)(64 * wpb));
   // a body force along one lattice axis takes the kernels formed for it
   int fax = -1;
