@@ -11,8 +11,15 @@
 //      plain load, non-temporal store;
 //   4  as 3 with non-temporal loads;
 //   5  as 3 with plain loads and stores.
+//
+// lbmpsm_pull_probe times the D3Q19 collide-stream's memory access pattern
+// alone: every stored node pulls its 19 populations from the neighbours
+// (blocked layout, 64-node blocks of 19 runs; periodic wrap on all axes)
+// and writes them to its own block of the other buffer, with no arithmetic.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "lbm_common.h"
 
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
@@ -141,4 +148,86 @@ extern "C" int lbmpsm_copy_probe_form(int form, int64_t bytes, int reps, double*
 
 extern "C" int lbmpsm_copy_probe(int64_t bytes, int reps, double* ms_per_copy) {
   return lbmpsm_copy_probe_form(0, bytes - bytes % 16, reps, ms_per_copy);
+}
+
+// One wave per 64-node chunk of a z-row (sz a multiple of 64).
+template <typename T>
+__global__ __launch_bounds__(64) void k_pull_pattern(const T* __restrict__ fin,
+                                                     T* __restrict__ fout, int sx, int sy, int sz) {
+  using V = lbmpsm::VelSet<19>;
+  const int nch = sz >> 6;
+  const int wave = (int)blockIdx.x;
+  const int row = wave / nch;
+  if (row >= sx * sy) return;
+  const int c = (wave - row * nch) * 64 + (int)threadIdx.x;
+  const int i = row / sy, j = row - i * sy;
+  T f[19];
+#pragma unroll
+  for (int q = 0; q < 19; ++q) {
+    int ii = i - V::ex[q], jj = j - V::ey[q], cc = c - V::ez[q];
+    ii += ii < 0 ? sx : 0;
+    ii -= ii >= sx ? sx : 0;
+    jj += jj < 0 ? sy : 0;
+    jj -= jj >= sy ? sy : 0;
+    cc += cc < 0 ? sz : 0;
+    cc -= cc >= sz ? sz : 0;
+    const size_t n = ((size_t)ii * sy + jj) * sz + cc;
+    f[q] = __builtin_nontemporal_load(fin + (n >> 6) * (19 * 64) + (n & 63) + q * 64);
+  }
+  const size_t n = ((size_t)i * sy + j) * sz + c;
+#pragma unroll
+  for (int q = 0; q < 19; ++q)
+    __builtin_nontemporal_store(f[q], fout + (n >> 6) * (19 * 64) + (n & 63) + q * 64);
+}
+
+template <typename T>
+static void launch_pull(const void* a, void* b, int sx, int sy, int sz) {
+  const int64_t waves = (int64_t)sx * sy * (sz >> 6);
+  hipLaunchKernelGGL(k_pull_pattern<T>, dim3((unsigned)waves), dim3(64), 0, 0, (const T*)a,
+                     (T*)b, sx, sy, sz);
+}
+
+// elem_bytes 8, 4 or 2 (the FP64, FP32 and FP16 storage forms); returns the
+// average time of one pass over `reps` launches after two untimed ones.
+extern "C" int lbmpsm_pull_probe(int sx, int sy, int sz, int elem_bytes, int reps,
+                                 double* ms_per_pass) {
+  if (sx <= 0 || sy <= 0 || sz <= 0 || sz % 64 || reps <= 0 || !ms_per_pass ||
+      (elem_bytes != 8 && elem_bytes != 4 && elem_bytes != 2))
+    return -1;
+  if ((int64_t)sx * sy * (sz >> 6) >= ((int64_t)1 << 31)) return -1;
+  const size_t bytes = (size_t)sx * sy * sz * 19 * elem_bytes;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess) return -2;
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    (void)hipFree(a);
+    return -2;
+  }
+  auto launch = [&](int r) {
+    const void* s = (r & 1) ? b : a;
+    void* d = (r & 1) ? a : b;
+    if (elem_bytes == 8) launch_pull<double>(s, d, sx, sy, sz);
+    else if (elem_bytes == 4) launch_pull<float>(s, d, sx, sy, sz);
+    else launch_pull<unsigned short>(s, d, sx, sy, sz);
+  };
+  int rc = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipMemset(a, 0, bytes) != hipSuccess || hipMemset(b, 0, bytes) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = -3;
+  } else {
+    for (int r = 0; r < 2; ++r) launch(r);
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; ++r) launch(r);
+    (void)hipEventRecord(e1, 0);
+    float ms = 0.f;
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = -3;
+    *ms_per_pass = ms / reps;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc;
 }

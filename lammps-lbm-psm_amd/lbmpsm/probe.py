@@ -13,6 +13,9 @@ def _lib():
     lib = C.CDLL(_PATH)
     lib.lbmpsm_copy_probe_form.restype = C.c_int
     lib.lbmpsm_copy_probe_form.argtypes = [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+    lib.lbmpsm_pull_probe.restype = C.c_int
+    lib.lbmpsm_pull_probe.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(C.c_double)]
     return lib
 
 
@@ -30,3 +33,15 @@ def copy_ceiling(nbytes=2 << 30, reps=10):
     res = {f: copy_bandwidth(nbytes, reps, f)[0] for f in FORMS}
     best = max(res, key=res.get)
     return res[best], best, res
+
+
+def pull_pattern(sx, sy, sz, elem_bytes, reps=10):
+    """Time (ms) of one pass of the D3Q19 collide-stream's access pattern
+    alone (19 pulled and 19 stored elements per node, no arithmetic) over an
+    sx x sy x sz box, sz a multiple of 64."""
+    ms = C.c_double()
+    rc = _lib().lbmpsm_pull_probe(int(sx), int(sy), int(sz), int(elem_bytes), int(reps),
+                                  C.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"pull probe failed ({rc})")
+    return ms.value

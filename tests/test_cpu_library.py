@@ -34,6 +34,18 @@ def test_abi_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_probe_library_exports_and_rejects_bad_shapes():
+    """The measurement library (copy ceiling, collide access pattern) loads
+    and checks its arguments before touching a device."""
+    from lbmpsm import probe
+    lib = probe._lib()
+    assert hasattr(lib, "lbmpsm_copy_probe_form") and hasattr(lib, "lbmpsm_pull_probe")
+    ms = C.c_double()
+    assert lib.lbmpsm_pull_probe(16, 16, 100, 8, 1, C.byref(ms)) == -1  # z not whole blocks
+    assert lib.lbmpsm_pull_probe(16, 16, 64, 3, 1, C.byref(ms)) == -1   # element size
+    assert lib.lbmpsm_copy_probe_form(6, 1 << 20, 1, C.byref(ms)) == -1  # no such form
+
+
 def _cfg(*args, **kw):
     return lbmpsm.config_from_fix(*args, **kw)
 
