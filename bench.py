@@ -44,12 +44,13 @@ KERNEL_NAME = {"f64": "k_collide_f64<3,19,false", "f32": "k_collide_lowp<3,19,1,
                "f16": "k_collide_lowp<3,19,2,false"}
 
 
-def kernel_name(storage, F):
+def kernel_name(storage, F, sz):
     """The collide kernel the library launches: formed for a body force along
-    one lattice axis (last template argument: that axis), else for any
-    force (-1)."""
+    one lattice axis (that axis), else for any force (-1); then whether the
+    stored z rows are a whole number of 64-node blocks (split addressing)."""
     nz = [a for a in range(3) if F[a] != 0.0]
-    return f"{KERNEL_NAME[storage]},{nz[0] if len(nz) == 1 else -1}>"
+    aln = "true" if sz % 64 == 0 else "false"
+    return f"{KERNEL_NAME[storage]},{nz[0] if len(nz) == 1 else -1},{aln}>"
 
 
 class Workload:
@@ -175,7 +176,7 @@ def roofline(wl, storage, res):
     achieved = bytes_per_update * nodes / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": kernel_name(storage, wl.F), "bytes_per_update": bytes_per_update,
+            "kernel": kernel_name(storage, wl.F, info.sz), "bytes_per_update": bytes_per_update,
             "nodes_per_launch": nodes, "avg_launch_ms": round(avg_ms, 5)}
     # HBM bytes per launch of the dominant kernel from the committed
     # rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy kernel)
@@ -337,6 +338,15 @@ def main():
     roof["copy_form"] = probe.FORMS[copy_form]
     roof["copy_forms"] = {probe.FORMS[f]: round(v, 1) for f, v in copy_all.items()}
     roof["frac_of_copy"] = round(roof["achieved"] / copy_gbs, 4)
+    # the collide's own access pattern with no arithmetic, at the stored box
+    info = main_res["info"]
+    if info.sz % 64 == 0:
+        print("[bench] pull pattern", file=sys.stderr, flush=True)
+        pat_ms = probe.pull_pattern(info.sx, info.sy, info.sz, BYTES_PER_ELEM[args.storage])
+        roof["pattern_ms"] = round(pat_ms, 5)
+        roof["pattern_gbs"] = round(roof["bytes_per_update"] * roof["nodes_per_launch"] /
+                                    (pat_ms * 1e-3) / 1e9, 1)
+        roof["frac_of_pattern"] = round(pat_ms / roof["avg_launch_ms"], 4)
     kernels = {k: {"avg_ms": round(v[0] / max(v[1], 1), 5), "launches": v[1]}
                for k, v in main_res["prof"].items()}
     variants = {}
