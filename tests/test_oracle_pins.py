@@ -449,3 +449,49 @@ def test_periodic_array_stokes_drag_hasimoto():
     K = Fo[0, 0] / (6.0 * math.pi * nu * a * U)
     assert abs(K / Kh - 1.0) < 0.05, (K, Kh)
     assert abs(Fo[0, 1]) < 1e-3 * Fo[0, 0] and abs(Fo[0, 2]) < 1e-3 * Fo[0, 0]
+
+
+def _halo_slot_mismatch(n, per, pos, radii, steps):
+    """Largest difference, over the steps, between the slots of a periodic
+    axis's halo planes and those of their interior counterparts (tags
+    compared as a count of differing slots), in the oracle's halo'd lattice."""
+    box = lattice_box(3, n, per)
+    atoms = random_atoms(box, radii, seed=29, positions=np.array(pos, float), vmax=1e-2,
+                         wmax=1e-3)
+    ora = O.Oracle(3, n, per, 0.7)
+    worst = 0.0
+    for step in range(steps):
+        H.move_linear(atoms, box, step, 1.0)
+        lst = H.build_list(atoms, box, float(np.max(atoms.radius)) + 4.0)
+        ora.fix_step(lst)
+        nh = ora.nh
+        sf = ora._view(O.LIB.lbo_slot_sf, ora.nodes_h * 2).reshape(*nh, 2)
+        tg = ora._view(O.LIB.lbo_slot_tag, ora.nodes_h * 2, np.int32).reshape(*nh, 2)
+        for ax in range(3):
+            if not per[ax]:
+                continue
+            m = nh[ax]
+            inner = tuple(slice(1, -1) if d != ax else slice(None) for d in range(3))
+            a_sf = np.take(sf, [0, m - 1], axis=ax)[inner]
+            b_sf = np.take(sf, [m - 2, 1], axis=ax)[inner]
+            a_tg = np.take(tg, [0, m - 1], axis=ax)[inner]
+            b_tg = np.take(tg, [m - 2, 1], axis=ax)[inner]
+            worst = max(worst, float(np.abs(a_sf - b_sf).max()), float((a_tg != b_tg).sum()))
+    return worst
+
+
+def test_periodic_image_zones_and_halo_slots():
+    """The reference keeps a halo on periodic axes and maps every list entry,
+    ghost images included, onto it, clearing a slot when an entry of the
+    same tag finds no overlap there (src/LBM_PSM_exchangeParticleData.cpp:
+    24-112).  When the map zones of one sphere's own images overlap (here
+    r = 3.2 across a 9-node periodic y extent, below 2r + 5), a later image
+    clears, on one side of the seam only, a slot an earlier image set: the
+    halo plane then ends the map with slots that differ from its periodic
+    counterpart, and the reference streams from it.  A lattice that keeps no
+    periodic halo, such as the library, streams from the counterpart instead.
+    With extents above 2r + 5 the halo planes equal their counterparts."""
+    pos = [[5.0, 4.4, 31.7], [4.6, 4.8, 2.1]]
+    assert _halo_slot_mismatch((10, 9, 64), (1, 1, 1), pos, [3.2, 2.8], 1) > 0
+    pos = [[7.0, 6.4, 31.7], [6.6, 6.8, 2.1]]
+    assert _halo_slot_mismatch((14, 13, 64), (1, 1, 1), pos, [3.2, 2.8], 4) == 0
