@@ -220,3 +220,48 @@ def test_reduced_storage_2d_channel(ref64_2d, storage, tol):
     print(f"  {storage}: drag {eF:.2e} torque {eT:.2e} velocity {eu:.2e} "
           f"mass {abs(m - m0) / m0:.2e}", flush=True)
     assert eF <= tol and eT <= tol and eu <= tol
+
+
+def run_large(storage, lists):
+    n = (256, 256, 256)
+    lat = lbmpsm.Lattice(lbmpsm.lattice_units_config(3, n, (1, 1, 1), 0.65, storage=storage,
+                                                     F_lbm=(5e-6, 0.0, 0.0)))
+    for lst in lists:
+        lat.set_particles(lst.tag, lst.x, lst.v, lst.omega, lst.radius)
+        lat.step()
+        F, T, _ = lat.hydro()
+    lat.close()
+    return F, T
+
+
+@pytest.fixture(scope="module")
+def large_lists():
+    from lbmpsm import workloads as W
+    box, atoms = W.dense_packing()
+    rng = np.random.default_rng(11)
+    atoms.v = rng.uniform(-1e-3, 1e-3, (atoms.n, 3))
+    atoms.omega = rng.uniform(-1e-4, 1e-4, (atoms.n, 3))
+    return W.atom_lists(box, atoms, 100, cutoff=14.0)
+
+
+@pytest.fixture(scope="module")
+def ref64_large(large_lists):
+    print(f"\n[case] precision reference FP64 256^3 periodic, {large_lists[-1].n} entries, "
+          f"100 steps", flush=True)
+    return run_large("f64", large_lists)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f16", 2e-3)])
+def test_reduced_storage_large_lattice(ref64_large, large_lists, storage, tol):
+    """256^3 periodic (262 144 row chunks: the size from which the reduced
+    collides run two-wave workgroups, dealt to the XCDs in contiguous
+    ranges), phi = 0.30 of spheres d ~ U[10, 20] moving with random
+    velocities, Fx = 5e-6, 100 steps: drag and torque against FP64."""
+    print(f"\n[case] precision {storage} 256^3", flush=True)
+    F, T = run_large(storage, large_lists)
+    F0, T0 = ref64_large
+    eF = np.max(np.linalg.norm(F - F0, axis=1)) / np.max(np.linalg.norm(F0, axis=1))
+    eT = np.max(np.linalg.norm(T - T0, axis=1)) / np.max(np.linalg.norm(T0, axis=1))
+    print(f"  {storage}: drag {eF:.2e} torque {eT:.2e}", flush=True)
+    assert eF <= tol and eT <= tol
