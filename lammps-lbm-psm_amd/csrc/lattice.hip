@@ -350,6 +350,9 @@ __device__ __forceinline__ void gather_row(const CollideArgs& a, const Row& r, i
     om = col_off<ST, Q>((unsigned)cm);
     op = col_off<ST, Q>((unsigned)cp);
   }
+  // every load unconditionally (src stays inside the buffer: nbr keeps the
+  // coordinate off a non-periodic edge), so that the loads of all
+  // directions are in flight together
   Unroll<0, Q>::run([&]<int q>() {
     constexpr int ex = VelSet<Q>::ex[q], ey = VelSet<Q>::ey[q], ez = VelSet<Q>::ez[q];
     // source node = node - e_q; the column axis is z in 3D and y in 2D
@@ -357,21 +360,29 @@ __device__ __forceinline__ void gather_row(const CollideArgs& a, const Row& r, i
     constexpr int eb = DIM == 3 ? ey : 0;
     constexpr int ec = DIM == 3 ? ez : ey;
     const unsigned col = (unsigned)(ec == 1 ? cm : (ec == -1 ? cp : c));
-    const bool cok = ec == 1 ? vcm : (ec == -1 ? vcp : true);
-    const bool rok = (r.okmask >> (3 * (1 - ea) + (1 - eb))) & 1u;
     const unsigned rs = row_base<DIM>(g, r, 1 - ea, 1 - eb);
     const T* src;
     if constexpr (AL) src = pop_ptr_al<ST, Q, q>(a.fin, rs, ec == 1 ? om : (ec == -1 ? op : oc));
     else src = pop_ptr<ST, Q>(a.fin, q, plane, rs, col);
-    // both values unconditionally (src stays inside the buffer: nbr keeps
-    // the coordinate off a non-periodic edge), then a select: with no branch
-    // around the load, the loads of all directions are in flight together
-    C cbv;
-    if constexpr (ST == ST_F64) cbv = a.cb[q];
-    else cbv = a.cbf[q];
-    const C v = ld<ST>(src, a.inv_scale);
-    f[q] = (rok && cok) ? v : cbv;
+    f[q] = ld<ST>(src, a.inv_scale);
   });
+  // then the wall default for a source off a non-periodic edge; a wave whose
+  // neighbour rows all exist, on a periodic column axis, has none and skips
+  // the selects (with the default in a scalar register, each also costs a
+  // move into a vector register)
+  if (r.okmask != 0x1FFu || !per_c) {
+    Unroll<0, Q>::run([&]<int q>() {
+      constexpr int ea = VelSet<Q>::ex[q];
+      constexpr int eb = DIM == 3 ? VelSet<Q>::ey[q] : 0;
+      constexpr int ec = DIM == 3 ? VelSet<Q>::ez[q] : VelSet<Q>::ey[q];
+      const bool cok = ec == 1 ? vcm : (ec == -1 ? vcp : true);
+      const bool rok = (r.okmask >> (3 * (1 - ea) + (1 - eb))) & 1u;
+      C cbv;
+      if constexpr (ST == ST_F64) cbv = a.cb[q];
+      else cbv = a.cbf[q];
+      f[q] = (rok && cok) ? f[q] : cbv;
+    });
+  }
   if constexpr (FACES) {
     const C r0 = (ST == ST_F64) ? C(0) : (C)a.rho0;
     const int jj = DIM == 3 ? r.j : c;
