@@ -616,26 +616,17 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
           // antisymmetric up to the sign of a zero), so the solid terms
           // reuse the fluid term's difference
           const double dneq = feq - fi;
-          double sc1 = 0.0, sc2 = 0.0;
+          // the reference's fi + (1 - Btot) dneq omega + B1 sc1 + B2 sc2,
+          // each slot's term added only where the slot is held: an empty
+          // slot has B = 0 and sc = 0, whose +0.0 could only change the
+          // sign of a zero
+          double out = fi + (1.0 - Btot) * dneq * omega;
+          // Node force on each slot, src/LBM_PSM_BGK_dynamics.h:101-113.
           if (o1) {
             const double e1 = edot<Q, q>(us1[0], us1[1], us1[2]);
             const double fs = rho * w * (1.0 + e1 * inv2 + e1 * e1 * hinv4 - u1sq);
-            sc1 = fs - fi + oneMinusOmega * -dneq;
-          }
-          if constexpr (TWO) {
-            if (o2) {
-              const double e2 = edot<Q, q>(us2[0], us2[1], us2[2]);
-              const double fs = rho * w * (1.0 + e2 * inv2 + e2 * e2 * hinv4 - u2sq);
-              sc2 = fs - fi + oneMinusOmega * -dneq;
-            }
-          }
-          double out = fi + (1.0 - Btot) * dneq * omega + B1 * sc1;
-          if constexpr (TWO) out = out + B2 * sc2;
-          if (FAX >= 0 || a.guo)
-            out = out + a.guo_pref * guo_f64<Q, q, FAX>(ux, uy, uz, eu, F0, F1, F2, inv2, inv4);
-          st<ST_F64>(dst.template operator()<q>(), out, 1.0f);
-          // Node force on each slot, src/LBM_PSM_BGK_dynamics.h:101-113.
-          if (o1) {
+            const double sc1 = fs - fi + oneMinusOmega * -dneq;
+            out = out + B1 * sc1;
             const double t = -B1 * sc1;
             if constexpr (cx > 0) Fa[0] = Fa[0] + t;
             if constexpr (cx < 0) Fa[0] = Fa[0] - t;
@@ -646,6 +637,10 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
           }
           if constexpr (TWO) {
             if (o2) {
+              const double e2 = edot<Q, q>(us2[0], us2[1], us2[2]);
+              const double fs = rho * w * (1.0 + e2 * inv2 + e2 * e2 * hinv4 - u2sq);
+              const double sc2 = fs - fi + oneMinusOmega * -dneq;
+              out = out + B2 * sc2;
               const double t = -B2 * sc2;
               if constexpr (cx > 0) Fa[3] = Fa[3] + t;
               if constexpr (cx < 0) Fa[3] = Fa[3] - t;
@@ -655,6 +650,9 @@ __device__ __forceinline__ void collide_f64(const CollideArgs& a, double* f, uns
               if constexpr (cz < 0) Fa[5] = Fa[5] - t;
             }
           }
+          if (FAX >= 0 || a.guo)
+            out = out + a.guo_pref * guo_f64<Q, q, FAX>(ux, uy, uz, eu, F0, F1, F2, inv2, inv4);
+          st<ST_F64>(dst.template operator()<q>(), out, 1.0f);
         });
       };
 #if LBMPSM_F64_SPLIT
